@@ -1,0 +1,94 @@
+/* walrus_hip.h -- C ABI of the MI355X power-trace hafnian library (libwalrus_hip.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved (re, im) doubles, byte-
+ * identical to numpy complex128; matrices are row-major.  Every function returns 0 on success
+ * and a negative code on error; wh_last_error() then describes the error for the calling
+ * thread.  Functions are thread-safe: each device's shared state is guarded by a lock.
+ *
+ * Reference seam: thewalrus has no FFI for this path; its kernel boundary is the pair of numba
+ * functions _calc_hafnian (thewalrus/_hafnian.py:417) and _calc_loop_hafnian
+ * (thewalrus/_hafnian.py:513), called by _haf (:507) and loop_hafnian (:630) on inputs that
+ * the Python frontends have already validated, paired (matched_reps, :80) and reordered.
+ * wh_calc_hafnian / wh_calc_loop_hafnian take exactly those prepared inputs.
+ */
+#ifndef WALRUS_HIP_H
+#define WALRUS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WH_MAX_EDGES 32 /* pairing edges per request: reordered matrix side 2E <= 64 */
+#define WH_MAX_PHOTONS 128 /* N = sum of repetitions handled on the device */
+
+/* Error codes. */
+#define WH_OK 0
+#define WH_EINVAL -1    /* invalid argument (size limits, null pointers, inconsistent job) */
+#define WH_EDEVICE -2   /* HIP runtime error */
+#define WH_ENOMEM -3    /* device or pinned allocation failed */
+#define WH_EUNSUPPORTED -4
+
+/* Human-readable description of the last error raised on the calling thread. */
+const char* wh_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int wh_version(void);
+
+/* Makes `device` current for this thread's later calls and creates its context (stream,
+ * workspaces).  Optional: the other entry points initialise device 0 (or the current device)
+ * on first use. */
+int wh_init(int device);
+
+/* Replaces _calc_hafnian(A, edge_reps, glynn) (thewalrus/_hafnian.py:417-467).
+ * A: host pointer, 2E x 2E complex128, rows/columns ordered by matched_reps (index i is
+ * paired with i + E).  edge_reps: E multiplicities (int64).  glynn: 1 finite-difference
+ * sieve, 0 inclusion-exclusion.  out: 2 doubles (re, im).  Synchronous. */
+int wh_calc_hafnian(const double* A, int64_t twoE, const int64_t* edge_reps, int64_t E,
+                    int glynn, double* out);
+
+/* Replaces _calc_loop_hafnian(A, D, edge_reps, oddloop, oddV, glynn)
+ * (thewalrus/_hafnian.py:513-577).  D: 2E complex loop weights (reordered).  oddloop: NULL,
+ * or a pointer to one complex (the weight of the unpaired mode, odd N); oddV: NULL, or 2E
+ * complex (that mode's reordered row).  Synchronous. */
+int wh_calc_loop_hafnian(const double* A, const double* D, const int64_t* edge_reps, int64_t E,
+                         const double* oddloop, const double* oddV, int glynn, double* out);
+
+/* One prepared request for the batched path.  Offsets count complex128 elements from the
+ * start of a device-resident pool that holds every request's reordered matrix (and vectors). */
+typedef struct wh_request {
+    int64_t a_off;       /* reordered 2E x 2E matrix */
+    int64_t d_off;       /* reordered loop weights (2E), or -1 for the plain hafnian */
+    int64_t v_off;       /* row of the unpaired mode (2E), or -1 when N is even */
+    double oddloop[2];   /* weight of the unpaired mode (odd N) */
+    int32_t E;           /* number of pairing edges, 1..WH_MAX_EDGES */
+    int32_t glynn;       /* 1: finite-difference sieve, 0: inclusion-exclusion */
+    int32_t real;        /* 1: matrix and weights have zero imaginary parts */
+    int32_t reserved;
+    int32_t edge_reps[WH_MAX_EDGES];
+} wh_request;
+
+typedef struct wh_plan wh_plan;
+
+/* Builds a launch plan for `n` requests on the current device: validates them, chooses the
+ * kernel per request and the term-chunk size (chunk_terms <= 0: automatic), and uploads the
+ * job table.  Allocates device memory; may block. */
+int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_plan** plan);
+
+/* Enqueues the plan on `stream` (a hipStream_t; NULL = the context stream): sieve kernels
+ * over device pool `dpool`, then the fold that writes n complex results (2n doubles) to
+ * device buffer `dout` in request order.  No allocation, no host synchronisation. */
+int wh_plan_run(wh_plan* plan, const double* dpool, double* dout, void* stream);
+
+/* Number of terms, workgroups, and algorithmic FP64 flops of the plan (products counted as
+ * the kernels run them).  Any pointer may be NULL. */
+int wh_plan_info(const wh_plan* plan, int64_t* terms, int64_t* blocks, double* flops);
+
+int wh_plan_destroy(wh_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WALRUS_HIP_H */

@@ -1,0 +1,234 @@
+"""CPU oracle for the power-trace hafnian -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+this module, and only as the checker (or as the timed CPU baseline).  The product package
+``thewalrus_amd`` never imports it.
+
+It wraps ``libwalrus_oracle.so`` (``walrus_oracle.cpp``, a C++ restatement of
+``thewalrus/_hafnian.py:26-631`` and ``thewalrus/charpoly.py:42-327``) and restates the
+pure-Python preparation of the reference (pairing, reorder) independently of the product's
+frontend, so the product's host code is checked against a second statement of the same rules.
+
+Precision argument: ``"double"`` follows the reference's operation order inside each term;
+``"dd"`` runs the same algorithm in double-double arithmetic and serves as high-precision
+truth.  ``variant=1`` swaps the per-term product loop for Hessenberg + La Budde + Newton
+identities (an independent route, cheaper by ~n/2, used for large-n truth).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    """Load (building first if needed) the oracle shared library."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libwalrus_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        L.wo_calc_sieve.argtypes = [dp, ctypes.c_int64, ip, ctypes.c_int64, dp, dp, dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.wo_powertrace.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp]
+        L.wo_charpoly.argtypes = [dp, ctypes.c_int64, dp]
+        L.wo_fcoeffs.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
+        L.wo_brute_force.argtypes = [dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
+        L.wo_kan.argtypes = [dp, dp, ip, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
+        _LIB = L
+    return _LIB
+
+
+def build():
+    """Compile the oracle library with its Makefile (g++ -fopenmp)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _unpack(out, precision):
+    if precision == 0:
+        return complex(out[0], out[1])
+    # double-double: keep as (value, low parts) -> python complex of the rounded sum, plus
+    # the exact pair for callers that need more digits
+    return complex(out[0] + out[1], out[2] + out[3])
+
+
+def _prec(precision):
+    return {"double": 0, "dd": 1, 0: 0, 1: 1}[precision]
+
+
+def calc_sieve(Ax, edge_reps, D=None, oddloop=None, oddV=None, glynn=True, precision="double",
+               nthreads=0, variant=0, exact_pair=False):
+    """Sieve at the reference kernel seam (``_hafnian.py:417`` / ``:513``)."""
+    p = _prec(precision)
+    Ax = _c(Ax)
+    e = np.ascontiguousarray(edge_reps, dtype=np.int64)
+    Dc = None if D is None else _c(D)
+    ol = None if oddloop is None else np.array([complex(oddloop)], dtype=np.complex128)
+    oV = None if oddV is None else _c(oddV)
+    out = np.zeros(4, dtype=np.float64)
+    rc = lib().wo_calc_sieve(_dp(Ax), Ax.shape[0], _ip(e), e.shape[0], _dp(Dc), _dp(ol), _dp(oV),
+                             int(bool(glynn)), p, int(nthreads), int(variant), _dp(out))
+    if rc != 0:
+        raise ValueError("oracle sieve rejected its input")
+    if exact_pair and p == 1:
+        return out.copy()
+    return _unpack(out, p)
+
+
+def powertrace(H, n, precision="double"):
+    """``charpoly.py:302`` powertrace(H, n) -> p[0..n-1]."""
+    p = _prec(precision)
+    H = _c(H)
+    m = H.shape[0]
+    out = np.zeros(4 * max(n, 1), dtype=np.float64)
+    lib().wo_powertrace(_dp(H), m, n, p, _dp(out))
+    if p == 0:
+        return out[: 2 * n].view(np.complex128).copy()
+    o = out.reshape(-1, 4)[:n]
+    return (o[:, 0] + o[:, 1]) + 1j * (o[:, 2] + o[:, 3])
+
+
+def charpoly(H):
+    """``charpoly.py:286`` charpoly(H) -> [c_1..c_m] of det(lambda I - H)."""
+    H = _c(H)
+    m = H.shape[0]
+    out = np.zeros(2 * m, dtype=np.float64)
+    lib().wo_charpoly(_dp(H), m, _dp(out))
+    return out.view(np.complex128).copy()
+
+
+def fcoeffs(AX, N, mode=0, XD=None, D=None, oddloop=None, oddVX=None):
+    """``f`` (mode 0), ``f_loop`` (1) or ``f_loop_odd`` (2) coefficient vectors."""
+    AX = _c(AX)
+    m = AX.shape[0]
+    out = np.zeros(2 * (N + 2), dtype=np.float64)
+    ol = None if oddloop is None else np.array([complex(oddloop)], dtype=np.complex128)
+    k = lib().wo_fcoeffs(_dp(AX), m, N, mode, _dp(None if XD is None else _c(XD)),
+                         _dp(None if D is None else _c(D)), _dp(ol),
+                         _dp(None if oddVX is None else _c(oddVX)), _dp(out))
+    return out[: 2 * k].view(np.complex128).copy()
+
+
+def brute_force(A, loop=False, precision="dd"):
+    """(Loop) hafnian by definition: sum over perfect matchings (with loops)."""
+    A = _c(A)
+    out = np.zeros(4, dtype=np.float64)
+    p = _prec(precision)
+    if lib().wo_brute_force(_dp(A), A.shape[0], int(bool(loop)), p, _dp(out)) != 0:
+        raise ValueError("brute force limited to n <= 30")
+    return _unpack(out, p)
+
+
+def kan(B, rpt, mu=None, loop=False, precision="dd"):
+    """Kan's repeated-moment formula (``docs/algorithms.rst:127-148``)."""
+    B = _c(B)
+    r = np.ascontiguousarray(rpt, dtype=np.int64)
+    u = None if mu is None else _c(mu)
+    out = np.zeros(4, dtype=np.float64)
+    p = _prec(precision)
+    lib().wo_kan(_dp(B), _dp(u), _ip(r), B.shape[0], int(bool(loop)), p, _dp(out))
+    return _unpack(out, p)
+
+
+# --------------------------------------------------------------------------------------
+# Restated host preparation (independent of the product's frontend)
+# --------------------------------------------------------------------------------------
+def pair_modes(reps):
+    """Restates ``matched_reps`` (``_hafnian.py:80-159``): greedy pairing of repeated modes.
+
+    Returns (x, edge_reps, oddmode).
+    """
+    reps = [int(r) for r in reps]
+    if sum(reps) == 0:
+        return np.array([], dtype=np.int64), np.array([], dtype=np.int64), None
+    items = sorted(((r, i) for i, r in enumerate(reps)), reverse=True)
+    items = [(r, i) for r, i in items if r > 0]
+    left, right, mult = [], [], []
+    while len(items) > 1 or (len(items) == 1 and items[0][0] > 1):
+        items.sort(reverse=True)
+        r0, i0 = items[0]
+        if len(items) == 1 or r0 > 2 * items[1][0]:
+            left.append(i0)
+            right.append(i0)
+            mult.append(r0 // 2)
+            if r0 % 2 == 0:
+                items = items[1:]
+            else:
+                items[0] = (1, i0)
+        else:
+            r1, i1 = items[1]
+            left.append(i0)
+            right.append(i1)
+            mult.append(r1)
+            if r0 > r1:
+                items = [(r0 - r1, i0)] + items[2:]
+            else:
+                items = items[2:]
+    odd = items[0][1] if len(items) == 1 else None
+    return np.array(left + right, dtype=np.int64), np.array(mult, dtype=np.int64), odd
+
+
+def hafnian_sieve(A, loop=False, rpt=None, mu=None, glynn=True, precision="double", nthreads=0,
+                  variant=0):
+    """End-to-end oracle value of ``_haf`` / ``loop_hafnian`` (``_hafnian.py:470-631``)."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    rpt = [1] * n if rpt is None else [int(r) for r in rpt]
+    N = sum(rpt)
+    if N == 0:
+        return 1.0
+    if not loop and N % 2 == 1:
+        return 0.0
+    D = np.diag(A).copy() if mu is None else np.asarray(mu)
+    if loop and N == 1:
+        return complex(D[rpt.index(1)])
+    x, e, odd = pair_modes(rpt)
+    Ax = A[np.ix_(x, x)].astype(np.complex128)
+    if not loop:
+        return calc_sieve(Ax, e, glynn=glynn, precision=precision, nthreads=nthreads,
+                          variant=variant)
+    Dx = D[x].astype(np.complex128)
+    oddloop = oddV = None
+    if odd is not None:
+        oddloop = complex(D[odd])
+        oddV = A[odd, x].astype(np.complex128)
+    return calc_sieve(Ax, e, D=Dx, oddloop=oddloop, oddV=oddV, glynn=glynn, precision=precision,
+                      nthreads=nthreads, variant=variant)
+
+
+def reduction(A, rpt):
+    """Row/column repetition (``_hafnian.py:697-714``)."""
+    rows = [i for i, r in enumerate(rpt) for _ in range(int(r))]
+    A = np.asarray(A)
+    if A.ndim == 1:
+        return A[rows]
+    return A[:, rows][rows]
+
+
+def double_factorial(n):
+    out = 1
+    for k in range(n, 0, -2):
+        out *= k
+    return out
+
+
+TELEPHONE = [1, 1, 2, 4, 10, 26, 76, 232, 764, 2620, 9496]  # test_hafnian.py:30

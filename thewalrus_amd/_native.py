@@ -1,0 +1,120 @@
+"""ctypes binding of ``libwalrus_hip.so`` (C ABI declared in ``include/walrus_hip.h``).
+
+The library is the only compute path: if it is missing, or no HIP device is usable, every
+entry point raises ``RuntimeError`` -- there is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwalrus_hip.so")
+
+MAX_EDGES = 32
+MAX_PHOTONS = 128
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+
+
+class WhRequest(ctypes.Structure):
+    """Mirror of ``wh_request`` in ``include/walrus_hip.h``."""
+
+    _fields_ = [
+        ("a_off", ctypes.c_int64),
+        ("d_off", ctypes.c_int64),
+        ("v_off", ctypes.c_int64),
+        ("oddloop", ctypes.c_double * 2),
+        ("E", ctypes.c_int32),
+        ("glynn", ctypes.c_int32),
+        ("real", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("edge_reps", ctypes.c_int32 * MAX_EDGES),
+    ]
+
+
+EXPORTS = [
+    "wh_last_error",
+    "wh_version",
+    "wh_init",
+    "wh_calc_hafnian",
+    "wh_calc_loop_hafnian",
+    "wh_plan_create",
+    "wh_plan_run",
+    "wh_plan_info",
+    "wh_plan_destroy",
+]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=None):
+    """Load the shared library and declare its signatures (no device call is made)."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(
+                "libwalrus_hip.so not found at %s: build it with `python -m thewalrus_amd._build`"
+                " (there is no CPU fallback)" % p
+            )
+        L = ctypes.CDLL(p)
+        L.wh_last_error.restype = ctypes.c_char_p
+        L.wh_last_error.argtypes = []
+        L.wh_version.restype = ctypes.c_int
+        L.wh_init.argtypes = [ctypes.c_int]
+        L.wh_calc_hafnian.argtypes = [_dp, ctypes.c_int64, _ip, ctypes.c_int64, ctypes.c_int, _dp]
+        L.wh_calc_loop_hafnian.argtypes = [_dp, _dp, _ip, ctypes.c_int64, _dp, _dp, ctypes.c_int, _dp]
+        L.wh_plan_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.POINTER(ctypes.c_void_p)]
+        L.wh_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_plan_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
+        L.wh_plan_destroy.argtypes = [ctypes.c_void_p]
+        for name in EXPORTS:
+            getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
+        if path is None:
+            _lib = L
+        return L
+
+
+def check(rc):
+    """Raise the library's error message for a nonzero return code."""
+    if rc != 0:
+        msg = _lib.wh_last_error().decode() if _lib is not None else "unknown error"
+        if rc == -1:
+            raise ValueError(msg)
+        raise RuntimeError("libwalrus_hip error %d: %s" % (rc, msg))
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def calc_hafnian(Ax, edge_reps, glynn=True):
+    """Device sieve at the reference seam ``_calc_hafnian`` (``_hafnian.py:417``)."""
+    L = load()
+    Ax = np.ascontiguousarray(Ax, dtype=np.complex128)
+    e = np.ascontiguousarray(edge_reps, dtype=np.int64)
+    out = np.zeros(2, dtype=np.float64)
+    check(L.wh_calc_hafnian(_dptr(Ax), Ax.shape[0], e.ctypes.data_as(_ip), e.shape[0],
+                            int(bool(glynn)), _dptr(out)))
+    return np.complex128(complex(out[0], out[1]))
+
+
+def calc_loop_hafnian(Ax, Dx, edge_reps, oddloop=None, oddV=None, glynn=True):
+    """Device sieve at the reference seam ``_calc_loop_hafnian`` (``_hafnian.py:513``)."""
+    L = load()
+    Ax = np.ascontiguousarray(Ax, dtype=np.complex128)
+    Dx = np.ascontiguousarray(Dx, dtype=np.complex128)
+    e = np.ascontiguousarray(edge_reps, dtype=np.int64)
+    ol = None if oddloop is None else np.array([complex(oddloop)], dtype=np.complex128)
+    oV = None if oddV is None else np.ascontiguousarray(oddV, dtype=np.complex128)
+    out = np.zeros(2, dtype=np.float64)
+    check(L.wh_calc_loop_hafnian(_dptr(Ax), _dptr(Dx), e.ctypes.data_as(_ip), e.shape[0], _dptr(ol),
+                                 _dptr(oV), int(bool(glynn)), _dptr(out)))
+    return np.complex128(complex(out[0], out[1]))

@@ -1,0 +1,401 @@
+// sieve_generic.hip -- general power-trace sieve: any edge multiplicities (repeated rows),
+// plain / loop / odd-N loop, Glynn or inclusion-exclusion.  One wave (64 lanes) owns one
+// workgroup and walks the terms of one chunk in order; every term's matrix, powers and
+// coefficient recursion live in LDS.  Restates, per term, _hafnian.py:443-462 (plain) and
+// :546-569 (loop), with powertrace / charpoly from charpoly.py:113-327.
+#include "wh_device.h"
+
+namespace {
+
+struct GenLds {
+    cplx* H;     // term matrix AX_S (m x m, row stride MM); Hessenberg form after charpoly
+    cplx* P;     // current power / La Budde coefficient table
+    cplx* Q;     // next power
+    cplx* XD;    // loop: delta-scaled X D
+    cplx* D;     // loop: D_S, later the column chain A^k D (odd N)
+    cplx* OV;    // odd: delta-scaled oddV X
+    cplx* v;     // work vector (Householder reflector, mat-vec chains)
+    cplx* w;     // work vector
+    cplx* pt;    // power traces p[0..h]
+    cplx* lv;    // loop scalars, index by order
+    cplx* ov;    // odd-loop scalars
+    cplx* comb;  // 2 x (WH_NMAX + 2) coefficient rows
+    int* S;      // kept edge indices
+    int* dS;     // their delta values
+};
+
+__device__ __forceinline__ GenLds carve(unsigned char* base, int MM) {
+    GenLds L;
+    cplx* c = reinterpret_cast<cplx*>(base);
+    size_t mm = (size_t)MM * MM;
+    L.H = c; c += mm;
+    L.P = c; c += mm;
+    L.Q = c; c += mm;
+    L.XD = c; c += MM;
+    L.D = c; c += MM;
+    L.OV = c; c += MM;
+    L.v = c; c += MM;
+    L.w = c; c += MM;
+    L.pt = c; c += WH_NMAX / 2 + 4;
+    L.lv = c; c += WH_NMAX / 2 + 4;
+    L.ov = c; c += WH_NMAX / 2 + 4;
+    L.comb = c; c += 2 * (WH_NMAX + 2);
+    L.S = reinterpret_cast<int*>(c);
+    L.dS = L.S + WH_EMAX;
+    return L;
+}
+
+// Q = P @ H (m x m), one output element per lane per step, k summed in order.
+__device__ void matmul_lds(const cplx* P, const cplx* H, cplx* Q, int m, int MM, int lane) {
+    for (int e = lane; e < m * m; e += WH_WAVE) {
+        int i = e / m, j = e - i * m;
+        cplx s = cmk(0.0, 0.0);
+        for (int k = 0; k < m; ++k) cmac(s, P[i * MM + k], H[k * MM + j]);
+        Q[i * MM + j] = s;
+    }
+}
+
+__device__ cplx trace_lds(const cplx* X, int m, int MM, int lane) {
+    cplx z = cmk(0.0, 0.0);
+    if (lane < m) z = X[lane * MM + lane];
+    return wave_csum(z);
+}
+
+// charpoly.py:113-125 + :43-109: in-place Householder reduction to upper Hessenberg form.
+__device__ void hessenberg_lds(GenLds& L, int m, int MM, int lane) {
+    cplx* H = L.H;
+    for (int k = 1; k < m - 1; ++k) {
+        int sizeH = m - k;
+        // reflection vector (charpoly.py:55-72)
+        cplx col = cmk(0.0, 0.0);
+        if (lane < sizeH) col = H[(k + lane) * MM + (k - 1)];
+        double nrm2 = wave_sum(lane < sizeH ? cabs2(col) : 0.0);
+        cplx c0 = H[k * MM + (k - 1)];
+        cplx sigma = cmk(sqrt(nrm2), 0.0);
+        if (c0.re != 0.0 || c0.im != 0.0) {
+            double a0 = sqrt(cabs2(c0));
+            sigma = cmul(sigma, cdivr(c0, a0));
+        }
+        if (lane < sizeH) L.v[lane] = (lane == 0) ? cadd(col, sigma) : col;
+        __syncthreads();
+        // apply_householder (charpoly.py:86-109)
+        double nv = sqrt(wave_sum(lane < sizeH ? cabs2(L.v[lane]) : 0.0));
+        double nv2 = nv * nv;
+        if (nv2 == 0.0) continue;
+        int ncol = m - k + 1;
+        if (lane < ncol) {
+            cplx s = cmk(0.0, 0.0);
+            for (int l = 0; l < sizeH; ++l) cmac(s, cconj(L.v[l]), H[(k + l) * MM + (k - 1 + lane)]);
+            L.w[lane] = s;
+        }
+        __syncthreads();
+        for (int e = lane; e < sizeH * ncol; e += WH_WAVE) {
+            int i = e / ncol, j = e - i * ncol;
+            cplx t = cdivr(cmul(cmulr(L.v[i], 2.0), L.w[j]), nv2);
+            cplx& a = H[(k + i) * MM + (k - 1 + j)];
+            a = csub(a, t);
+        }
+        __syncthreads();
+        cplx av = cmk(0.0, 0.0);
+        if (lane < m)
+            for (int l = 0; l < sizeH; ++l) cmac(av, H[lane * MM + (k + l)], L.v[l]);
+        __syncthreads();
+        if (lane < m) L.w[lane] = av;
+        __syncthreads();
+        for (int e = lane; e < m * sizeH; e += WH_WAVE) {
+            int i = e / sizeH, j = e - i * sizeH;
+            cplx t = cdivr(cmul(cmulr(L.w[i], 2.0), cconj(L.v[j])), nv2);
+            cplx& a = H[i * MM + (k + j)];
+            a = csub(a, t);
+        }
+        __syncthreads();
+    }
+}
+
+// charpoly.py:196-282 La Budde on the Hessenberg matrix in L.H; coefficient table in L.P.
+// Row i of the table depends only on earlier rows, so the lanes fill one row in parallel.
+__device__ void labudde_lds(GenLds& L, int m, int MM, int lane) {
+    const cplx* H = L.H;
+    cplx* C = L.P;
+#define HH(i, j) H[((i) - 1) * MM + ((j) - 1)]
+#define CC(i, j) C[((i) - 1) * MM + ((j) - 1)]
+    if (lane == 0) {
+        CC(1, 1) = cneg(HH(1, 1));
+        if (m >= 2) {
+            CC(2, 1) = csub(CC(1, 1), HH(2, 2));
+            CC(2, 2) = csub(cmul(HH(1, 1), HH(2, 2)), cmul(HH(1, 2), HH(2, 1)));
+        }
+    }
+    __syncthreads();
+    for (int i = 3; i <= m; ++i) {
+        int j = lane + 1;
+        if (j <= i) {
+            cplx alpha = HH(i, i);
+            if (j == 1) {
+                CC(i, 1) = csub(CC(i - 1, 1), alpha);
+            } else if (j < i) {
+                cplx suma = cmk(0.0, 0.0), bp = cmk(1.0, 0.0);
+                for (int mm = 1; mm < j - 1; ++mm) {
+                    bp = cmul(bp, HH(i - mm + 1, i - mm));
+                    suma = cadd(suma, cmul(cmul(HH(i - mm, i), bp), CC(i - mm - 1, j - mm - 1)));
+                }
+                cplx bpj = cmk(1.0, 0.0);
+                for (int bm = i; bm > i - j + 1; --bm) bpj = cmul(bpj, HH(bm, bm - 1));
+                cplx v = csub(CC(i - 1, j), cmul(alpha, CC(i - 1, j - 1)));
+                v = csub(v, suma);
+                CC(i, j) = csub(v, cmul(HH(i - j + 1, i), bpj));
+            } else {
+                cplx suma = cmk(0.0, 0.0), bp = cmk(1.0, 0.0);
+                for (int mm = 1; mm < i - 1; ++mm) {
+                    bp = cmul(bp, HH(i - mm + 1, i - mm));
+                    suma = cadd(suma, cmul(cmul(HH(i - mm, i), bp), CC(i - mm - 1, i - mm - 1)));
+                }
+                cplx bpi = cmk(1.0, 0.0);
+                for (int bm = i; bm > 1; --bm) bpi = cmul(bpi, HH(bm, bm - 1));
+                cplx v = csub(cneg(cmul(alpha, CC(i - 1, i - 1))), suma);
+                CC(i, i) = csub(v, cmul(HH(1, i), bpi));
+            }
+        }
+        __syncthreads();
+    }
+#undef HH
+#undef CC
+}
+
+}  // namespace
+
+// jobs[]: sorted by chunk_base; block b finds its job by binary search.
+// partials[b] = (sum_re, err_re, sum_im, err_im) of the chunk's signed terms.
+extern "C" __global__ void __launch_bounds__(WH_WAVE)
+wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
+                 double4* __restrict__ partials, int MM) {
+    extern __shared__ __align__(16) unsigned char wh_smem[];
+    const int lane = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
+    }
+    const wh_job_dev& J = jobs[lo];
+    const int E = J.E, twoE = 2 * E, N = J.N;
+    const bool loop = (J.flags & WH_F_LOOP) != 0;
+    const bool odd = (J.flags & WH_F_ODD) != 0;
+    const bool glynn = (J.flags & WH_F_GLYNN) != 0;
+    const int h = N / 2;
+    const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
+    int64_t t1 = t0 + J.chunk_terms;
+    if (t1 > J.steps) t1 = J.steps;
+    const cplx* A = pool + J.a_off;
+    const cplx* Dg = loop ? pool + J.d_off : nullptr;
+    const cplx* Vg = odd ? pool + J.v_off : nullptr;
+    const cplx oddloop = cmk(J.oddloop_re, J.oddloop_im);
+    GenLds L = carve(wh_smem, MM);
+
+    // per-lane constants for the mixed-radix decode (lane i < E owns digit i)
+    int my_e = lane < E ? J.e[lane] : 0;
+    int64_t my_w = 1;  // product of (e_l + 1) over l > lane
+    for (int l = lane + 1; l < E; ++l) my_w *= (int64_t)(J.e[l] + 1);
+    const int top = odd ? N : h;           // coefficient index taken from f
+    const int acc_lane = top % WH_WAVE;    // lane that keeps the running sum
+    csum acc;
+    acc.zero();
+
+    for (int64_t t = t0; t < t1; ++t) {
+        // _hafnian.py:444-459: digits, binomial product, sign, delta
+        int k = 0;
+        if (lane < E) k = (int)((t / my_w) % (int64_t)(my_e + 1));
+        int delta = glynn ? 2 * k - my_e : k;
+        int edge_sum = k;
+        double bin = lane < E ? binom_d(my_e, k) : 1.0;
+#pragma unroll
+        for (int msk = 1; msk < WH_WAVE; msk <<= 1) {
+            edge_sum += __shfl_xor(edge_sum, msk, WH_WAVE);
+            bin *= __shfl_xor(bin, msk, WH_WAVE);  // exact: products of small integers
+        }
+        int delta0 = __shfl(delta, 0, WH_WAVE);
+        unsigned long long nz = __ballot(lane < E && delta != 0);
+        int s = __popcll(nz);
+        int m = 2 * s;
+        if (lane < E && delta != 0) {
+            int pos = __popcll(nz & ((1ull << lane) - 1ull));
+            L.S[pos] = lane;
+            L.dS[pos] = delta;
+        }
+        double prefac = (((h - edge_sum) & 1) ? -1.0 : 1.0) * bin;
+        if (!odd && glynn && delta0 == 0) prefac *= 0.5;
+        __syncthreads();
+
+        // _hafnian.py:289-356: gather the kept rows/columns, swap halves, scale columns
+        for (int e = lane; e < m * m; e += WH_WAVE) {
+            int r = e / m, c = e - r * m;
+            int rr = r < s ? L.S[r] : L.S[r - s] + E;
+            int cc, d;
+            if (c < s) { cc = L.S[c] + E; d = L.dS[c]; }
+            else { cc = L.S[c - s]; d = L.dS[c - s]; }
+            L.H[r * MM + c] = cmulr(A[(int64_t)rr * twoE + cc], (double)d);
+        }
+        if (loop && lane < m) {
+            int rr = lane < s ? L.S[lane] : L.S[lane - s] + E;
+            L.D[lane] = Dg[rr];
+            int c = lane < s ? lane : lane - s;
+            int src = lane < s ? L.S[c] + E : L.S[c];
+            double d = (double)L.dS[c];
+            L.XD[lane] = cmulr(Dg[src], d);
+            if (odd) L.OV[lane] = cmulr(Vg[src], d);
+        }
+        __syncthreads();
+
+        // loop scalars, taken before powertrace may overwrite H (f_loop uses the unmutated AX_S)
+        if (loop && !odd) {
+            // _hafnian.py:232-234: a_i gets XD A^(i-1) D / 2; XD <- XD @ AX_S (row chain)
+            if (lane < m) L.v[lane] = L.XD[lane];
+            __syncthreads();
+            for (int i = 1; i <= h; ++i) {
+                cplx part = (lane < m) ? cmul(L.v[lane], L.D[lane]) : cmk(0.0, 0.0);
+                cplx dot = wave_csum(part);
+                if (lane == 0) L.lv[i] = dot;
+                cplx nv = cmk(0.0, 0.0);
+                if (lane < m)
+                    for (int r = 0; r < m; ++r) cmac(nv, L.v[r], L.H[r * MM + lane]);
+                __syncthreads();
+                if (lane < m) L.v[lane] = nv;
+                __syncthreads();
+            }
+        } else if (odd) {
+            // _hafnian.py:268-275: even orders use XD . z, odd orders oddVX . z, z <- AX_S z
+            if (lane < m) L.v[lane] = L.D[lane];
+            __syncthreads();
+            for (int kk = 1; kk <= h; ++kk) {
+                cplx a = (lane < m) ? cmul(L.XD[lane], L.v[lane]) : cmk(0.0, 0.0);
+                cplx o = (lane < m) ? cmul(L.OV[lane], L.v[lane]) : cmk(0.0, 0.0);
+                cplx da = wave_csum(a), dob = wave_csum(o);
+                if (lane == 0) { L.lv[kk] = da; L.ov[kk] = dob; }
+                cplx nv = cmk(0.0, 0.0);
+                if (lane < m)
+                    for (int c = 0; c < m; ++c) cmac(nv, L.H[lane * MM + c], L.v[c]);
+                __syncthreads();
+                if (lane < m) L.v[lane] = nv;
+                __syncthreads();
+            }
+        }
+
+        // charpoly.py:302-327 powertrace(H, h + 1)
+        const int nt = h + 1;
+        if (m == 0) {
+            for (int kk = lane; kk <= h; kk += WH_WAVE) L.pt[kk] = cmk(0.0, 0.0);
+        } else {
+            int min_val = nt < m ? nt : m;
+            cplx tr = trace_lds(L.H, m, MM, lane);
+            if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
+            const cplx* cur = L.H;
+            for (int it = 0; it < min_val - 2; ++it) {
+                cplx* dst = (cur == L.P) ? L.Q : L.P;
+                matmul_lds(cur, L.H, dst, m, MM, lane);
+                __syncthreads();
+                cplx tk = trace_lds(dst, m, MM, lane);
+                if (lane == 0) L.pt[it + 2] = tk;
+                cur = dst;
+            }
+            __syncthreads();
+            if (nt > m) {
+                hessenberg_lds(L, m, MM, lane);
+                labudde_lds(L, m, MM, lane);
+                if (lane == 0) {
+                    // Cayley-Hamilton extension: p_t = -sum_k c_k p_(t-1-k)
+                    for (int tt = min_val; tt < nt; ++tt) {
+                        cplx sacc = cmk(0.0, 0.0);
+                        for (int q = 0; q < m; ++q)
+                            sacc = csub(sacc, cmul(L.P[(m - 1) * MM + q], L.pt[tt - 1 - q]));
+                        L.pt[tt] = sacc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // _hafnian.py:196-208 (f), :228-241 (f_loop), :264-283 (f_loop_odd)
+        cplx* oldc = L.comb;
+        cplx* newc = L.comb + (WH_NMAX + 2);
+        for (int kk = lane; kk <= top; kk += WH_WAVE) oldc[kk] = (kk == 0) ? cmk(1.0, 0.0) : cmk(0.0, 0.0);
+        __syncthreads();
+        for (int i = 1; i <= top; ++i) {
+            cplx factor;
+            if (!odd) {
+                factor = cdivr(L.pt[i], (double)(2 * i));
+                if (loop) factor = cadd(factor, cdivr(L.lv[i], 2.0));
+            } else if (i == 1) {
+                factor = oddloop;
+            } else if ((i & 1) == 0) {
+                factor = cadd(cdivr(L.pt[i / 2], (double)i), cdivr(L.lv[i / 2], 2.0));
+            } else {
+                factor = L.ov[(i - 1) / 2];
+            }
+            for (int kk = lane; kk <= top; kk += WH_WAVE) {
+                cplx val = oldc[kk];
+                cplx pf = cmk(1.0, 0.0);
+                for (int j = 1; j <= top / i; ++j) {
+                    pf = cmul(pf, cdivr(factor, (double)j));
+                    if (i * j <= kk) val = cadd(val, cmul(oldc[kk - i * j], pf));
+                }
+                newc[kk] = val;
+            }
+            __syncthreads();
+            cplx* tmp = oldc; oldc = newc; newc = tmp;
+        }
+        if (lane == acc_lane) acc.add(cmulr(oldc[top], prefac));
+        __syncthreads();
+    }
+    if (lane == acc_lane) partials[b] = make_double4(acc.sr, acc.cr, acc.si, acc.ci);
+}
+
+// Folds each job's chunk partials in chunk order with compensation and applies the sieve's
+// power-of-two scale (_hafnian.py:464-465, :571-575).  One thread per job.
+extern "C" __global__ void wh_fold(const wh_job_dev* __restrict__ jobs, int njobs,
+                                   const double4* __restrict__ partials, double2* __restrict__ out) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= njobs) return;
+    const wh_job_dev& J = jobs[q];
+    csum acc;
+    acc.zero();
+    for (int c = 0; c < J.nchunks; ++c) {
+        double4 p = partials[J.chunk_base + c];
+        acc.add_pair(p.x, p.y, p.z, p.w);
+    }
+    double re = acc.sr + acc.cr, im = acc.si + acc.ci;
+    if (J.flags & WH_F_GLYNN) {
+        int ex = (J.flags & WH_F_ODD) ? J.N / 2 : J.N / 2 - 1;
+        double sc = ldexp(1.0, -ex);
+        re *= sc;
+        im *= sc;
+    }
+    out[J.out_index] = make_double2(re, im);
+}
+
+// ---- host-side launchers (kernels are launched from the translation unit that defines them)
+#include "wh_launch.h"
+
+size_t wh_generic_lds_bytes(int MM) {
+    size_t c = 3 * (size_t)MM * MM + 5 * (size_t)MM + 3 * (WH_NMAX / 2 + 4) + 2 * (WH_NMAX + 2);
+    return c * sizeof(cplx) + 2 * WH_EMAX * sizeof(int);
+}
+
+hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                             int64_t nblocks, int MM, hipStream_t stream) {
+    size_t lds = wh_generic_lds_bytes(MM);
+    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_generic,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(wh_sieve_generic, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
+                       njobs, (const cplx*)pool, (double4*)partials, MM);
+    return hipGetLastError();
+}
+
+hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partials, void* out,
+                          hipStream_t stream) {
+    int threads = 64;
+    int blocks = (njobs + threads - 1) / threads;
+    hipLaunchKernelGGL(wh_fold, dim3(blocks), dim3(threads), 0, stream, jobs, njobs,
+                       (const double4*)partials, (double2*)out);
+    return hipGetLastError();
+}

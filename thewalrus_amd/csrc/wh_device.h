@@ -1,0 +1,102 @@
+// wh_device.h -- device-side types and helpers shared by the sieve kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WH_WAVE 64
+#define WH_EMAX 32    // max pairing edges per job (matrix side 2E <= 64)
+#define WH_NMAX 128   // max photon number N = sum(rpt) handled on the device
+
+// One hafnian request as the kernels see it: the reordered matrix Ax (2E x 2E, row-major,
+// interleaved complex128), optional loop weights, edge multiplicities and the term range.
+// It is the device form of the arguments of _calc_hafnian / _calc_loop_hafnian
+// (thewalrus/_hafnian.py:417, :513).
+struct __attribute__((aligned(16))) wh_job_dev {
+    int64_t a_off;        // element offset of Ax in the complex pool
+    int64_t d_off;        // element offset of D (loop) or -1
+    int64_t v_off;        // element offset of oddV (odd-N loop) or -1
+    int64_t steps;        // number of sieve terms
+    int64_t chunk_base;   // index of this job's first partial-sum slot
+    int64_t chunk_terms;  // terms per chunk (one workgroup per chunk)
+    int64_t out_index;    // position of the result in the caller's output array
+    double oddloop_re, oddloop_im;
+    int32_t E, N;         // edges, photons (N = 2 sum(e) [+1 if odd])
+    int32_t flags;        // WH_F_* bits
+    int32_t nchunks;
+    int32_t e[WH_EMAX];   // edge multiplicities
+};
+
+#define WH_F_LOOP 1
+#define WH_F_ODD 2
+#define WH_F_GLYNN 4
+#define WH_F_REAL 8
+
+struct cplx {
+    double re, im;
+};
+
+__device__ __forceinline__ cplx cmk(double r, double i) { cplx z; z.re = r; z.im = i; return z; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cmk(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return cmk(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ cplx cneg(cplx a) { return cmk(-a.re, -a.im); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ __forceinline__ cplx cmulr(cplx a, double d) { return cmk(a.re * d, a.im * d); }
+__device__ __forceinline__ cplx cdivr(cplx a, double d) { return cmk(a.re / d, a.im / d); }
+__device__ __forceinline__ cplx cconj(cplx a) { return cmk(a.re, -a.im); }
+__device__ __forceinline__ double cabs2(cplx a) { return a.re * a.re + a.im * a.im; }
+// acc += a * b
+__device__ __forceinline__ void cmac(cplx& acc, cplx a, cplx b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+
+// Error-free sum (Knuth TwoSum): s + err == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& err) {
+    s = a + b;
+    double bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated (Neumaier-style) complex accumulator for the alternating-sign term sum.
+struct csum {
+    double sr, si, cr, ci;
+    __device__ __forceinline__ void zero() { sr = si = cr = ci = 0.0; }
+    __device__ __forceinline__ void add(cplx x) {
+        double s, e;
+        two_sum(sr, x.re, s, e); sr = s; cr += e;
+        two_sum(si, x.im, s, e); si = s; ci += e;
+    }
+    __device__ __forceinline__ void add_pair(double r, double rc, double i, double ic) {
+        double s, e;
+        two_sum(sr, r, s, e); sr = s; cr += e + rc;
+        two_sum(si, i, s, e); si = s; ci += e + ic;
+    }
+};
+
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) {
+    return __shfl_xor(v, mask, WH_WAVE);
+}
+// Butterfly sum over the whole wave; every lane receives the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < WH_WAVE; m <<= 1) v += shfl_xor_d(v, m);
+    return v;
+}
+__device__ __forceinline__ cplx wave_csum(cplx z) { return cmk(wave_sum(z.re), wave_sum(z.im)); }
+
+// Binomial coefficient as the reference's table holds it (_hafnian.py:27-45), in double.
+__device__ __forceinline__ double binom_d(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    if (k == 0 || k == n) return 1.0;
+    int kk = k < n - k ? k : n - k;
+    int64_t b = 1;
+    for (int i = 0; i < kk; ++i) {
+        b *= (n - i);
+        b /= (i + 1);
+    }
+    return (double)b;
+}

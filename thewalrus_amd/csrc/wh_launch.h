@@ -1,0 +1,13 @@
+// wh_launch.h -- host-side launchers exported by the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+struct wh_job_dev;
+
+size_t wh_generic_lds_bytes(int MM);
+hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                             int64_t nblocks, int MM, hipStream_t stream);
+hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partials, void* out,
+                          hipStream_t stream);
