@@ -65,9 +65,13 @@ typedef struct wh_request {
     int32_t E;           /* number of pairing edges, 1..WH_MAX_EDGES */
     int32_t glynn;       /* 1: finite-difference sieve, 0: inclusion-exclusion */
     int32_t real;        /* 1: matrix and weights have zero imaginary parts */
-    int32_t reserved;
+    int32_t options;     /* WH_OPT_* bits */
     int32_t edge_reps[WH_MAX_EDGES];
 } wh_request;
+
+/* wh_request.options: run the request on the general kernel even where the specialised
+ * all-ones kernel applies (used to cross-check the two kernels on identical inputs). */
+#define WH_OPT_GENERIC 1
 
 typedef struct wh_plan wh_plan;
 

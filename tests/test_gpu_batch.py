@@ -1,0 +1,117 @@
+"""GPU tests of the batched path: kernel cross-checks, batch = per-call, large structured cases.
+
+Tolerance: relative error <= 1e-10 against double-double truth; bitwise equality where the
+same request runs through the same kernel and chunking (alone, or inside a batch).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import random_symmetric, relerr
+
+import thewalrus_amd as tw
+from thewalrus_amd import batch as B
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _mixed(rng, sizes, loops):
+    out = []
+    for n, lp in zip(sizes, loops):
+        out.append((random_symmetric(rng, n, scale=1.0 / np.sqrt(2 * n)), lp))
+    return out
+
+
+@pytest.mark.parametrize("loop", [False, True])
+@pytest.mark.parametrize("real", [False, True])
+def test_pm1_matches_generic_and_truth(loop, real):
+    rng = np.random.default_rng(11 + 2 * loop + real)
+    sizes = [6, 8, 10, 12, 14, 16, 18, 20]
+    reqs = [(random_symmetric(rng, n, complex_=not real, scale=1 / np.sqrt(n)), loop) for n in sizes]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    fast = B.DeviceBatch(preps)
+    gen = B.DeviceBatch(preps, force_generic=True)
+    fast.run()
+    gen.run()
+    rf, rg = fast.results(), gen.results()
+    for (A, l), a, g in zip(reqs, rf, rg):
+        truth = O.hafnian_sieve(A, loop=l, precision="dd")
+        assert relerr(a, truth) < TOL, (A.shape[0], a, truth)
+        assert relerr(g, truth) < TOL, (A.shape[0], g, truth)
+
+
+def test_batch_equals_per_call_bitwise():
+    """Each result is the same alone (per call) and inside a mixed batch (same bits)."""
+    rng = np.random.default_rng(5)
+    sizes = [8, 24, 10, 22, 12, 20, 14, 18, 16, 26, 28]
+    reqs = _mixed(rng, sizes, [i % 2 == 0 for i in range(len(sizes))])
+    reqs.append((np.ones((24, 24)), False))  # real input, specialised real kernel
+    res = B.hafnian_batch([A for A, _ in reqs], loop=[l for _, l in reqs])
+    for (A, l), r in zip(reqs, res):
+        alone = tw.hafnian(A, loop=l)
+        assert complex(alone) == complex(r), (A.shape[0], l, alone, r)
+    # the same requests in a different order and mix give the same bits
+    perm = rng.permutation(len(reqs))
+    res2 = B.hafnian_batch([reqs[i][0] for i in perm], loop=[reqs[i][1] for i in perm])
+    for k, i in enumerate(perm):
+        assert complex(res2[k]) == complex(res[i])
+
+
+def test_all_ones_24_in_batch():
+    """16 all-ones 24x24 inside a batch of random ones: 23!! = 316234143225 exactly."""
+    rng = np.random.default_rng(138)
+    M = rng.uniform(-1, 1, size=(48, 24, 24))
+    mats = [(m + m.T) / 2 for m in M] + [np.ones((24, 24))] * 16
+    res = B.hafnian_batch(mats)
+    for r in res[-16:]:
+        assert abs(r.real - 316234143225) < 0.5 and round(r.real) == 316234143225
+    for A, r in list(zip(mats, res))[:4]:
+        truth = O.hafnian_sieve(A, precision="dd")
+        assert relerr(r, truth) < TOL
+
+
+def _structured(n_half, loop, seed):
+    """A = W P (A1 (+) A2) P^T W with known (loop) hafnian lhaf(A1) lhaf(A2) prod(w).
+
+    Each vertex is covered once, by an edge (weight w_i w_j) or by its loop, so the loop
+    weights scale by w_i (not w_i^2)."""
+    rng = np.random.default_rng(seed)
+    A1 = random_symmetric(rng, n_half, scale=1 / np.sqrt(2 * n_half))
+    A2 = random_symmetric(rng, n_half, scale=1 / np.sqrt(2 * n_half))
+    if loop:
+        for X in (A1, A2):
+            X[np.diag_indices(n_half)] = (rng.normal(size=n_half) + 1j * rng.normal(size=n_half)) / np.sqrt(2)
+    n = 2 * n_half
+    A = np.zeros((n, n), dtype=np.complex128)
+    A[:n_half, :n_half] = A1
+    A[n_half:, n_half:] = A2
+    perm = rng.permutation(n)
+    A = A[np.ix_(perm, perm)]
+    w = np.exp(1j * rng.uniform(0, 2 * np.pi, size=n)) * rng.uniform(0.8, 1.25, size=n)
+    d = np.diag(A).copy()
+    A = (w[:, None] * A) * w[None, :]
+    A[np.diag_indices(n)] = w * d
+    t1 = O.hafnian_sieve(A1, loop=loop, precision="dd")
+    t2 = O.hafnian_sieve(A2, loop=loop, precision="dd")
+    return A, t1 * t2 * np.prod(w)
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_structured_n40(loop):
+    """n = 40 single solve (2^19 Glynn terms) against an exact product of two n = 20 truths."""
+    A, truth = _structured(20, loop, 39 + loop)
+    got = tw.hafnian(A, loop=loop)
+    assert relerr(got, truth) < TOL, (got, truth)
+
+
+def test_structured_n32_loop_batch():
+    mats, truths = [], []
+    for s in range(4):
+        A, t = _structured(16, True, 100 + s)
+        mats.append(A)
+        truths.append(t)
+    res = B.hafnian_batch(mats, loop=True)
+    for r, t in zip(res, truths):
+        assert relerr(r, t) < TOL

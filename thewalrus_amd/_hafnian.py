@@ -80,11 +80,33 @@ def matched_reps(reps):
     return x, np.asarray(mult, dtype=np.int64), oddmode
 
 
-def _haf(A, reps=None, glynn=True):
-    """Hafnian with optional repeated rows/columns (``_hafnian.py:470-508``).
+class SieveJob:
+    """Sieve inputs at the reference kernel seam (``_calc_hafnian`` / ``_calc_loop_hafnian``).
 
-    Pairs the modes, reorders ``A`` and evaluates the sieve on the GPU.
+    ``Ax``: reordered matrix (complex128, 2E x 2E); ``edge_reps``: multiplicities;
+    ``Dx``/``oddloop``/``oddV``: loop weights (``Dx is None`` for the plain hafnian).
     """
+
+    __slots__ = ("Ax", "edge_reps", "Dx", "oddloop", "oddV", "glynn")
+
+    def __init__(self, Ax, edge_reps, Dx=None, oddloop=None, oddV=None, glynn=True):
+        self.Ax, self.edge_reps, self.Dx = Ax, edge_reps, Dx
+        self.oddloop, self.oddV, self.glynn = oddloop, oddV, glynn
+
+    @property
+    def loop(self):
+        return self.Dx is not None
+
+    def run(self):
+        """Evaluate on the GPU (synchronous, host buffers)."""
+        if self.Dx is None:
+            return _native.calc_hafnian(self.Ax, self.edge_reps, self.glynn)
+        return _native.calc_loop_hafnian(self.Ax, self.Dx, self.edge_reps, self.oddloop, self.oddV,
+                                         self.glynn)
+
+
+def _prepare_haf(A, reps=None, glynn=True):
+    """Host part of ``_haf`` (``_hafnian.py:486-505``): a value, or the sieve job."""
     n = A.shape[0]
     if reps is None:
         reps = [1] * n
@@ -96,14 +118,11 @@ def _haf(A, reps=None, glynn=True):
     assert n == len(reps)
     x, edge_reps, _ = matched_reps(reps)
     Ax = A[np.ix_(x, x)].astype(np.complex128)
-    return _native.calc_hafnian(Ax, edge_reps, glynn)
+    return SieveJob(Ax, edge_reps, glynn=glynn)
 
 
-def loop_hafnian(A, D=None, reps=None, glynn=True):
-    """Loop hafnian with optional repeated rows/columns (``_hafnian.py:581-631``).
-
-    ``D`` replaces the diagonal (defaults to ``A``'s diagonal).
-    """
+def _prepare_loop(A, D=None, reps=None, glynn=True):
+    """Host part of ``loop_hafnian`` (``_hafnian.py:598-628``): a value, or the sieve job."""
     n = A.shape[0]
     if reps is None:
         reps = [1] * n
@@ -125,11 +144,50 @@ def loop_hafnian(A, D=None, reps=None, glynn=True):
         oddV = None
     Ax = A[np.ix_(x, x)].astype(np.complex128)
     Dx = D[x].astype(np.complex128)
-    return _native.calc_loop_hafnian(Ax, Dx, edge_reps, oddloop, oddV, glynn)
+    return SieveJob(Ax, edge_reps, Dx, oddloop, oddV, glynn)
+
+
+def _finish(r):
+    return r.run() if isinstance(r, SieveJob) else r
+
+
+def _haf(A, reps=None, glynn=True):
+    """Hafnian with optional repeated rows/columns (``_hafnian.py:470-508``).
+
+    Pairs the modes, reorders ``A`` and evaluates the sieve on the GPU.
+    """
+    return _finish(_prepare_haf(A, reps, glynn))
+
+
+def loop_hafnian(A, D=None, reps=None, glynn=True):
+    """Loop hafnian with optional repeated rows/columns (``_hafnian.py:581-631``).
+
+    ``D`` replaces the diagonal (defaults to ``A``'s diagonal).
+    """
+    return _finish(_prepare_loop(A, D, reps, glynn))
 
 
 def hafnian(A, loop=False, rtol=1e-05, atol=1e-08, approx=False, num_samples=1000, method="glynn"):
     """Returns the (loop) hafnian of ``A`` (``_hafnian.py:718-818``).
+
+    Args:
+        A (array): square, symmetric array
+        loop (bool): return the loop hafnian
+        rtol, atol (float): tolerances of the symmetry check
+        approx (bool): Barvinok estimator -- not part of this library (raises)
+        num_samples (int): samples of the estimator
+        method (str): ``"glynn"`` (finite-difference sieve) or ``"inclexcl"``
+            (inclusion-exclusion); ``"recursive"`` is not part of this library
+
+    Returns:
+        int or float or complex: the hafnian (numpy complex128 from the sieve)
+    """
+    return _finish(prepare_hafnian(A, loop, rtol, atol, approx, num_samples, method))
+
+
+def prepare_hafnian(A, loop=False, rtol=1e-05, atol=1e-08, approx=False, num_samples=1000,
+                    method="glynn"):
+    """Host part of :func:`hafnian`: validation and shortcuts; a value or a :class:`SieveJob`.
 
     Args:
         A (array): square, symmetric array
@@ -185,12 +243,12 @@ def hafnian(A, loop=False, rtol=1e-05, atol=1e-08, approx=False, num_samples=100
     if loop:
         if method == "recursive":
             warnings.warn("Recursive algorithm does not support the loop hafnian")
-        return loop_hafnian(A, D=None, reps=None, glynn=True)
+        return _prepare_loop(A, D=None, reps=None, glynn=True)
     if method == "recursive":
         raise NotImplementedError(
             "method='recursive' is outside this library's power-trace path; use 'glynn'"
         )
-    return _haf(A, reps=None, glynn=glynn)
+    return _prepare_haf(A, reps=None, glynn=glynn)
 
 
 def hafnian_repeated(A, rpt, mu=None, loop=False, rtol=1e-05, atol=1e-08, glynn=True):
@@ -199,6 +257,11 @@ def hafnian_repeated(A, rpt, mu=None, loop=False, rtol=1e-05, atol=1e-08, glynn=
     Equals ``hafnian(reduction(A, rpt))``; with ``loop=True`` the diagonal is ``mu``
     (default: ``A``'s diagonal).
     """
+    return _finish(prepare_hafnian_repeated(A, rpt, mu, loop, rtol, atol, glynn))
+
+
+def prepare_hafnian_repeated(A, rpt, mu=None, loop=False, rtol=1e-05, atol=1e-08, glynn=True):
+    """Host part of :func:`hafnian_repeated`: a value or a :class:`SieveJob`."""
     # pylint: disable=too-many-return-statements,too-many-branches
     input_validation(A, atol=atol, rtol=rtol)
     if len(rpt) != len(A):
@@ -219,5 +282,5 @@ def hafnian_repeated(A, rpt, mu=None, loop=False, rtol=1e-05, atol=1e-08, glynn=
     if len(mu) != len(A):
         raise ValueError("Length of means vector must be the same length as the matrix A.")
     if loop:
-        return loop_hafnian(A, D=mu, reps=rpt, glynn=glynn)
-    return _haf(A, reps=rpt, glynn=glynn)
+        return _prepare_loop(A, D=mu, reps=rpt, glynn=glynn)
+    return _prepare_haf(A, reps=rpt, glynn=glynn)

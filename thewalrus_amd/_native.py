@@ -30,7 +30,7 @@ class WhRequest(ctypes.Structure):
         ("E", ctypes.c_int32),
         ("glynn", ctypes.c_int32),
         ("real", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("options", ctypes.c_int32),
         ("edge_reps", ctypes.c_int32 * MAX_EDGES),
     ]
 

@@ -148,58 +148,161 @@ int describe(const wh_request& r, wh_job_dev& j) {
     return WH_OK;
 }
 
-int64_t auto_chunk(int64_t steps) {
-    // a chunk is walked by one wave: small jobs get one term per wave, large ones enough
-    // terms to amortise the chunk's fixed cost while keeping thousands of workgroups
-    int64_t c = steps / 256;
+// Kernel families.  PM1: Glynn, even N, every multiplicity 1 (m = n for every term); the
+// register-tiled kernel with T x T tiles per lane, n_pad = 8T.  GENERIC: everything else.
+enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1 };
+constexpr int kPm1MaxT = 6;
+
+struct GroupKey {
+    int kind, T, real, loop;
+    bool operator<(const GroupKey& o) const {
+        if (kind != o.kind) return kind < o.kind;
+        if (T != o.T) return T < o.T;
+        if (real != o.real) return real < o.real;
+        return loop < o.loop;
+    }
+    bool operator==(const GroupKey& o) const {
+        return kind == o.kind && T == o.T && real == o.real && loop == o.loop;
+    }
+};
+
+GroupKey classify(const wh_job_dev& j, bool force_generic) {
+    GroupKey g{KIND_GENERIC, 0, 0, 0};
+    if (force_generic) return g;
+    bool all_one = true;
+    for (int i = 0; i < j.E; ++i) all_one = all_one && j.e[i] == 1;
+    bool glynn = (j.flags & WH_F_GLYNN) != 0, odd = (j.flags & WH_F_ODD) != 0;
+    int T = (2 * j.E + 7) / 8;
+    if (all_one && glynn && !odd && T <= kPm1MaxT && j.E >= 2) {
+        g.kind = KIND_PM1;
+        g.T = T;
+        g.real = (j.flags & WH_F_REAL) ? 1 : 0;
+        g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
+    }
+    return g;
+}
+
+// Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
+// under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
+int pm1_waves(int T, bool real) {
+    const size_t lds_cap = 160 * 1024;
+    int best = 1, best_w = 0;
+    for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
+        size_t b = wh_pm1_lds_bytes(T, real, nw);
+        if (b > lds_cap) break;
+        int per_cu = (int)(lds_cap / b) * nw;
+        if (per_cu > 16) per_cu = 16;
+        if (per_cu > best_w) { best_w = per_cu; best = nw; }
+    }
+    return best;
+}
+
+int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
+    if (g.kind == KIND_PM1) {
+        int64_t c = j.steps / 1024;
+        if (c < 4 * nw) c = 4 * nw;
+        if (c > 4096) c = 4096;
+        c = (c + nw - 1) / nw * nw;
+        return c;
+    }
+    // generic: one wave per workgroup walks the chunk
+    int64_t c = j.steps / 256;
     if (c < 1) c = 1;
     if (c > 256) c = 256;
     return c;
 }
 
-// Orders the kernel groups, chunks each job, and reports the number of workgroups.
+double term_flops(const wh_job_dev& j, const GroupKey& g) {
+    // algorithmic FP64 flops of the products per term, as the kernels run them:
+    // PM1: h - 1 products of n x n (8 flops per complex multiply-add, 2 per real one);
+    // GENERIC: (min(h + 1, m) - 2) products of m x m with m = 2 * nnz(delta) <= 2E (upper bound)
+    int h = j.N / 2;
+    int m = 2 * j.E;
+    int np = std::min(h + 1, m) - 2;
+    if (np < 0) np = 0;
+    double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;
+    return c * m * (double)m * m * np;
+}
+
+struct Group {
+    GroupKey key;
+    int64_t job_begin = 0, job_end = 0;
+    int64_t block_begin = 0, block_end = 0;
+    int nw = 1;
+    int MM = 0;  // generic: LDS matrix side
+};
+
 struct Layout {
-    std::vector<wh_job_dev> jobs;
-    int64_t n_generic = 0;
-    int64_t blocks_generic = 0;
-    int MM = 0;
+    std::vector<wh_job_dev> jobs;  // sorted by group
+    std::vector<Group> groups;
+    int64_t blocks = 0;
     int64_t terms = 0;
     double flops = 0.0;
 };
 
-double term_flops(const wh_job_dev& j, int m) {
-    // products as the generic kernel runs them: (min(h+1, m) - 2) complex m^3 products
-    int h = j.N / 2;
-    int np = std::min(h + 1, m) - 2;
-    if (np < 0) np = 0;
-    return 8.0 * m * (double)m * m * np;
-}
-
 int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& L) {
-    L.jobs.resize(n);
+    std::vector<wh_job_dev> tmp(n);
+    std::vector<GroupKey> keys(n);
     for (int64_t i = 0; i < n; ++i) {
-        int rc = describe(reqs[i], L.jobs[i]);
+        int rc = describe(reqs[i], tmp[i]);
         if (rc) return rc;
-        L.jobs[i].out_index = i;
-        int side = 2 * reqs[i].E;
-        if (side > kGenericMaxSide)
-            return fail(WH_EUNSUPPORTED, "matrix side %d exceeds the generic kernel limit %d", side,
-                        kGenericMaxSide);
-        L.MM = std::max(L.MM, side);
+        tmp[i].out_index = i;
+        keys[i] = classify(tmp[i], (reqs[i].options & WH_OPT_GENERIC) != 0);
+        if (keys[i].kind == KIND_GENERIC && 2 * reqs[i].E > kGenericMaxSide)
+            return fail(WH_EUNSUPPORTED, "matrix side %d exceeds the generic kernel limit %d",
+                        2 * reqs[i].E, kGenericMaxSide);
     }
+    std::vector<int64_t> order(n);
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+    L.jobs.resize(n);
     int64_t base = 0;
-    for (auto& j : L.jobs) {
-        int64_t c = chunk_hint > 0 ? chunk_hint : auto_chunk(j.steps);
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t i = order[k];
+        const GroupKey& g = keys[i];
+        if (L.groups.empty() || !(L.groups.back().key == g)) {
+            Group gr;
+            gr.key = g;
+            gr.job_begin = k;
+            gr.block_begin = base;
+            gr.nw = g.kind == KIND_PM1 ? pm1_waves(g.T, g.real != 0) : 1;
+            L.groups.push_back(gr);
+        }
+        Group& gr = L.groups.back();
+        wh_job_dev j = tmp[i];
+        int64_t c = chunk_hint > 0 ? chunk_hint : chunk_for(j, g, gr.nw);
         j.chunk_terms = c;
         j.nchunks = (int32_t)((j.steps + c - 1) / c);
         j.chunk_base = base;
         base += j.nchunks;
+        gr.job_end = k + 1;
+        gr.block_end = base;
+        if (g.kind == KIND_GENERIC) gr.MM = std::max(gr.MM, 2 * j.E);
         L.terms += j.steps;
-        L.flops += (double)j.steps * term_flops(j, 2 * j.E);
+        L.flops += (double)j.steps * term_flops(j, g);
+        L.jobs[k] = j;
     }
-    L.n_generic = n;
-    L.blocks_generic = base;
+    L.blocks = base;
     return WH_OK;
+}
+
+// Enqueues the sieve kernels of every group and the fold.
+hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* pool, double4* partials,
+                      void* out, hipStream_t s) {
+    for (const Group& g : L.groups) {
+        const wh_job_dev* gj = d_jobs + g.job_begin;
+        int nj = (int)(g.job_end - g.job_begin);
+        int64_t nb = g.block_end - g.block_begin;
+        hipError_t e;
+        if (g.key.kind == KIND_PM1)
+            e = wh_launch_pm1(g.key.T, g.key.real != 0, g.key.loop != 0, gj, nj, pool, partials, nb,
+                              g.block_begin, g.nw, s);
+        else
+            e = wh_launch_generic(gj, nj, pool, partials, nb, g.block_begin, g.MM, s);
+        if (e != hipSuccess) return e;
+    }
+    return wh_launch_fold(d_jobs, (int)L.jobs.size(), partials, out, s);
 }
 
 }  // namespace
@@ -241,7 +344,7 @@ int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_pl
     if (rc) { delete p; return rc; }
     const Layout& L = p->layout;
     if (hipMalloc((void**)&p->d_jobs, L.jobs.size() * sizeof(wh_job_dev)) != hipSuccess ||
-        hipMalloc((void**)&p->d_partials, std::max<int64_t>(L.blocks_generic, 1) * sizeof(double4)) !=
+        hipMalloc((void**)&p->d_partials, std::max<int64_t>(L.blocks, 1) * sizeof(double4)) !=
             hipSuccess) {
         wh_plan_destroy(p);
         return fail(WH_ENOMEM, "plan allocation failed");
@@ -264,18 +367,14 @@ int wh_plan_run(wh_plan* p, const double* dpool, double* dout, void* stream) {
         if (rc) return rc;
         s = c->stream;
     }
-    const Layout& L = p->layout;
-    if (L.n_generic > 0)
-        HIP_TRY(wh_launch_generic(p->d_jobs, (int)L.n_generic, dpool, p->d_partials, L.blocks_generic,
-                                  L.MM, s));
-    HIP_TRY(wh_launch_fold(p->d_jobs, (int)L.jobs.size(), p->d_partials, dout, s));
+    HIP_TRY(run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s));
     return WH_OK;
 }
 
 int wh_plan_info(const wh_plan* p, int64_t* terms, int64_t* blocks, double* flops) {
     if (!p) return fail(WH_EINVAL, "null plan");
     if (terms) *terms = p->layout.terms;
-    if (blocks) *blocks = p->layout.blocks_generic;
+    if (blocks) *blocks = p->layout.blocks;
     if (flops) *flops = p->layout.flops;
     return WH_OK;
 }
@@ -313,6 +412,14 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
         if (edge_reps[i] < 0 || edge_reps[i] > (1 << 20)) return fail(WH_EINVAL, "edge multiplicity out of range");
         r.edge_reps[i] = (int32_t)edge_reps[i];
     }
+    {
+        // zero imaginary parts everywhere: the real-arithmetic kernels compute the same values
+        bool real = true;
+        for (size_t i = 0; i < (size_t)side * side && real; ++i) real = A[2 * i + 1] == 0.0;
+        for (int64_t i = 0; D && i < side && real; ++i) real = D[2 * i + 1] == 0.0;
+        if (oddV || oddloop) real = false;
+        r.real = real ? 1 : 0;
+    }
     // staging layout (complex elements): A at 0, D after it, oddV after that
     size_t nA = (size_t)side * side;
     r.a_off = 0;
@@ -327,7 +434,7 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
     size_t bytes = job_bytes + data_elems * 16;
     rc = ensure_stage(*c, bytes);
     if (rc) return rc;
-    rc = ensure_partials(*c, (size_t)std::max<int64_t>(L.blocks_generic, 1));
+    rc = ensure_partials(*c, (size_t)std::max<int64_t>(L.blocks, 1));
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));  // staging is reused: previous copies done
     memcpy(c->h_stage, L.jobs.data(), sizeof(wh_job_dev));
@@ -338,8 +445,7 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
     HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
     const wh_job_dev* djob = (const wh_job_dev*)c->d_stage;
     const void* dpool = c->d_stage + job_bytes;
-    HIP_TRY(wh_launch_generic(djob, 1, dpool, c->d_partials, L.blocks_generic, L.MM, c->stream));
-    HIP_TRY(wh_launch_fold(djob, 1, c->d_partials, c->d_out, c->stream));
+    HIP_TRY(run_layout(L, djob, dpool, c->d_partials, c->d_out, c->stream));
     HIP_TRY(hipMemcpyAsync(c->h_out, c->d_out, sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     out[0] = c->h_out[0];

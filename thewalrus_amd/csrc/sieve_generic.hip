@@ -168,10 +168,10 @@ __device__ void labudde_lds(GenLds& L, int m, int MM, int lane) {
 // partials[b] = (sum_re, err_re, sum_im, err_im) of the chunk's signed terms.
 extern "C" __global__ void __launch_bounds__(WH_WAVE)
 wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
-                 double4* __restrict__ partials, int MM) {
+                 double4* __restrict__ partials, int64_t block_offset, int MM) {
     extern __shared__ __align__(16) unsigned char wh_smem[];
     const int lane = threadIdx.x;
-    const int64_t b = blockIdx.x;
+    const int64_t b = blockIdx.x + block_offset;
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -381,13 +381,13 @@ size_t wh_generic_lds_bytes(int MM) {
 }
 
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                             int64_t nblocks, int MM, hipStream_t stream) {
+                             int64_t nblocks, int64_t block_offset, int MM, hipStream_t stream) {
     size_t lds = wh_generic_lds_bytes(MM);
     hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_generic,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(wh_sieve_generic, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
-                       njobs, (const cplx*)pool, (double4*)partials, MM);
+                       njobs, (const cplx*)pool, (double4*)partials, block_offset, MM);
     return hipGetLastError();
 }
 
