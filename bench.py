@@ -51,9 +51,9 @@ def w_glynn(n, real=False):
     return 2 ** (h - 1) * (2.0 if real else 8.0) * n ** 3 * max(h - 1, 0)
 
 
-def timed_runs(batch, k, torch):
-    """Run the plan k times between two synchronisations; HIP-event and wall times (s)."""
-    s = torch.cuda.current_stream()
+def timed_runs(batch, k, torch, s):
+    """Run the plan k times on stream s between two synchronisations; HIP-event time of the
+    kernels on s and wall time (s)."""
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -126,10 +126,11 @@ def main():
     preps = [B.prepare(A, loop=l) for A, l in reqs]
     batch = B.DeviceBatch(preps)
 
+    stream = torch.cuda.Stream()  # default priority; the kernels and the events share it
     for _ in range(args.warmup):
-        batch.run()
+        batch.run(stream)
     torch.cuda.synchronize()
-    ev_s, wall = timed_runs(batch, args.steps, torch)
+    ev_s, wall = timed_runs(batch, args.steps, torch, stream)
     value = len(reqs) * args.steps / wall
     res = batch.results()
 
@@ -142,9 +143,9 @@ def main():
         if not idx:
             continue
         sub = B.DeviceBatch([preps[i] for i in idx])
-        sub.run()
+        sub.run(stream)
         torch.cuda.synchronize()
-        t, _ = timed_runs(sub, 1, torch)
+        t, _ = timed_runs(sub, 1, torch, stream)
         fl = sub.flops
         if best is None or t > best[0]:
             best = (t, fl, want_loop, len(idx))

@@ -72,6 +72,10 @@ typedef struct wh_request {
 /* wh_request.options: run the request on the general kernel even where the specialised
  * all-ones kernel applies (used to cross-check the two kernels on identical inputs). */
 #define WH_OPT_GENERIC 1
+/* wh_request.options: on the all-ones kernel, form every power B^2..B^h and take traces as the
+ * reference does (h - 1 products per term) instead of the default half-products formulation
+ * (powers up to B^ceil(h/2), traces of higher powers as <B^a, (B^b)^T>). */
+#define WH_OPT_FULL 2
 
 typedef struct wh_plan wh_plan;
 
@@ -80,7 +84,7 @@ typedef struct wh_plan wh_plan;
  * job table.  Allocates device memory; may block. */
 int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_plan** plan);
 
-/* Enqueues the plan on `stream` (a hipStream_t; NULL = the context stream): sieve kernels
+/* Enqueues the plan on `stream` (a hipStream_t; NULL = the default stream): sieve kernels
  * over device pool `dpool`, then the fold that writes n complex results (2n doubles) to
  * device buffer `dout` in request order.  No allocation, no host synchronisation. */
 int wh_plan_run(wh_plan* plan, const double* dpool, double* dout, void* stream);
@@ -88,6 +92,15 @@ int wh_plan_run(wh_plan* plan, const double* dpool, double* dout, void* stream);
 /* Number of terms, workgroups, and algorithmic FP64 flops of the plan (products counted as
  * the kernels run them).  Any pointer may be NULL. */
 int wh_plan_info(const wh_plan* plan, int64_t* terms, int64_t* blocks, double* flops);
+
+/* Copies the plan's per-workgroup partial sums to host memory after the last run (waits for
+ * the device): 4 doubles per slot, (sum_re, err_re, sum_im, err_im), the compensated sum of the
+ * signed terms of one chunk before the power-of-two scale.  With chunk_terms = 1 every slot is
+ * one term.  Diagnostic. */
+int wh_plan_partials(const wh_plan* plan, double* host_out);
+
+/* Per request (request order): first partial slot, number of slots, terms per slot. */
+int wh_plan_chunks(const wh_plan* plan, int64_t* first_chunk, int64_t* nchunks, int64_t* chunk_terms);
 
 int wh_plan_destroy(wh_plan* plan);
 

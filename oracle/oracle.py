@@ -36,6 +36,8 @@ def lib():
         L.wo_calc_sieve.argtypes = [dp, ctypes.c_int64, ip, ctypes.c_int64, dp, dp, dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.wo_powertrace.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp]
+        L.wo_sieve_terms.argtypes = [dp, ctypes.c_int64, ip, ctypes.c_int64, dp, dp, dp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.wo_charpoly.argtypes = [dp, ctypes.c_int64, dp]
         L.wo_fcoeffs.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         L.wo_brute_force.argtypes = [dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
@@ -92,6 +94,24 @@ def calc_sieve(Ax, edge_reps, D=None, oddloop=None, oddV=None, glynn=True, preci
     if exact_pair and p == 1:
         return out.copy()
     return _unpack(out, p)
+
+
+def sieve_terms(Ax, edge_reps, steps, D=None, oddloop=None, oddV=None, glynn=True,
+                precision="double", nthreads=0):
+    """Every signed term (prefactor * f) of the sieve, in term order (before the final scale)."""
+    p = _prec(precision)
+    Ax = _c(Ax)
+    e = np.ascontiguousarray(edge_reps, dtype=np.int64)
+    Dc = None if D is None else _c(D)
+    ol = None if oddloop is None else np.array([complex(oddloop)], dtype=np.complex128)
+    oV = None if oddV is None else _c(oddV)
+    out = np.zeros(4, dtype=np.float64)
+    terms = np.zeros(2 * steps, dtype=np.float64)
+    rc = lib().wo_sieve_terms(_dp(Ax), Ax.shape[0], _ip(e), e.shape[0], _dp(Dc), _dp(ol), _dp(oV),
+                              int(bool(glynn)), p, int(nthreads), _dp(out), _dp(terms))
+    if rc != 0:
+        raise ValueError("oracle sieve rejected its input")
+    return terms.view(np.complex128)
 
 
 def powertrace(H, n, precision="double"):

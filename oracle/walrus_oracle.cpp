@@ -504,7 +504,7 @@ static cx<T> term_value(TermMats<T>& tm, int N, bool loop, const cx<T>* oddloop,
 template <class T>
 static cx<T> calc_sieve(const cvec<T>& A, int twoE, const int64_t* e, int E, const cvec<T>* D,
                         const cx<T>* oddloop, const cvec<T>* oddV, bool glynn, int nthreads,
-                        int variant) {
+                        int variant, double* terms_out = nullptr) {
     int64_t esum = 0, emax = 0;
     for (int i = 0; i < E; ++i) { esum += e[i]; emax = std::max(emax, e[i]); }
     int N = (int)(2 * esum) + (oddloop ? 1 : 0);
@@ -542,6 +542,11 @@ static cx<T> calc_sieve(const cvec<T>& A, int twoE, const int64_t* e, int E, con
             if (!oddloop && glynn && delta[0] == 0) prefac *= 0.5;  // :458-459, :565-566
             cx<T> fval = term_value(tm, N, loop, oddloop, variant);
             Hloc += mulr(fval, T(prefac));
+            if (terms_out) {
+                cx<T> tv = mulr(fval, T(prefac));
+                terms_out[2 * j] = to_double(tv.re);
+                terms_out[2 * j + 1] = to_double(tv.im);
+            }
         }
         partial[tid] = Hloc;
     }
@@ -650,7 +655,7 @@ static void store(const cx<dd>& z, double* out) {
 template <class T>
 static void sieve_entry(const double* A, int64_t twoE, const int64_t* e, int64_t E, const double* D,
                         const double* oddloop, const double* oddV, int glynn, int nthreads, int variant,
-                        double* out) {
+                        double* out, double* terms_out = nullptr) {
     cvec<T> Av = load<T>(A, (size_t)twoE * twoE);
     cvec<T> Dv, oV;
     cx<T> ol;
@@ -658,7 +663,7 @@ static void sieve_entry(const double* A, int64_t twoE, const int64_t* e, int64_t
     if (oddloop) ol = cx<T>(T(oddloop[0]), T(oddloop[1]));
     if (oddV) oV = load<T>(oddV, twoE);
     cx<T> H = calc_sieve<T>(Av, (int)twoE, e, (int)E, D ? &Dv : nullptr, oddloop ? &ol : nullptr,
-                            oddV ? &oV : nullptr, glynn != 0, nthreads, variant);
+                            oddV ? &oV : nullptr, glynn != 0, nthreads, variant, terms_out);
     store(H, out);
 }
 
@@ -672,6 +677,17 @@ int wo_calc_sieve(const double* A, int64_t twoE, const int64_t* e, int64_t E, co
     if (twoE != 2 * E || E < 1) return -1;
     if (precision == 0) sieve_entry<double>(A, twoE, e, E, D, oddloop, oddV, glynn, nthreads, variant, out);
     else sieve_entry<dd>(A, twoE, e, E, D, oddloop, oddV, glynn, nthreads, variant, out);
+    return 0;
+}
+
+// Same sieve, also writing every signed term (prefactor * f) to terms_out (2 doubles per term,
+// in term order), before the final power-of-two scale.
+int wo_sieve_terms(const double* A, int64_t twoE, const int64_t* e, int64_t E, const double* D,
+                   const double* oddloop, const double* oddV, int glynn, int precision, int nthreads,
+                   double* out, double* terms_out) {
+    if (twoE != 2 * E || E < 1) return -1;
+    if (precision == 0) sieve_entry<double>(A, twoE, e, E, D, oddloop, oddV, glynn, nthreads, 0, out, terms_out);
+    else sieve_entry<dd>(A, twoE, e, E, D, oddloop, oddV, glynn, nthreads, 0, out, terms_out);
     return 0;
 }
 

@@ -1,0 +1,112 @@
+"""Writes the golden fixtures used by the GPU parity tests (run on the CPU, in this repo).
+
+    python tests/golden/make_golden.py small   # seconds: configs 1, 2, 4, 5 samples
+    python tests/golden/make_golden.py n40     # minutes: config 3 and structured n = 40
+
+Truth values come from the oracle's double-double restatement (precision="dd"); for n = 40
+the per-term characteristic-polynomial route (variant=1) keeps the cost to minutes.  The
+oracle's FP64 restatement of the reference algorithm (precision="double", variant=0) is
+stored beside each truth, so the tests can compare the device's error with the reference
+algorithm's own FP64 error on the same input.  Structured n = 40 cases carry an exact truth
+(product of two n = 20 values, see ``structured``).  Files are plain .npz (no pickles).
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+from thewalrus_amd import workloads as W  # noqa: E402
+
+
+def _c(z):
+    return np.complex128(z)
+
+
+def structured(n_half, loop, seed):
+    """A = W P (A1 (+) A2) P^T W: lhaf(A) = lhaf(A1) lhaf(A2) prod(w).  Off-diagonal entries
+    scale by w_i w_j, loop weights by w_i (each vertex is covered once)."""
+    rng = np.random.default_rng(seed)
+    s = np.sqrt(1.0 / (4 * n_half))
+    blocks = []
+    for _ in range(2):
+        B = rng.normal(scale=s, size=(n_half, n_half)) + 1j * rng.normal(scale=s, size=(n_half, n_half))
+        X = (B + B.T) / 2
+        if loop:
+            X[np.diag_indices(n_half)] = (rng.normal(size=n_half) + 1j * rng.normal(size=n_half)) / np.sqrt(2)
+        blocks.append(X)
+    n = 2 * n_half
+    A = np.zeros((n, n), dtype=np.complex128)
+    A[:n_half, :n_half], A[n_half:, n_half:] = blocks
+    perm = rng.permutation(n)
+    A = A[np.ix_(perm, perm)]
+    w = np.exp(1j * rng.uniform(0, 2 * np.pi, size=n)) * rng.uniform(0.8, 1.25, size=n)
+    d = np.diag(A).copy()
+    A = (w[:, None] * A) * w[None, :]
+    A[np.diag_indices(n)] = w * d
+    t = _c(O.hafnian_sieve(blocks[0], loop=loop, precision="dd")) * _c(
+        O.hafnian_sieve(blocks[1], loop=loop, precision="dd")
+    ) * np.prod(w)
+    return A, t
+
+
+def small():
+    out = {}
+    A = W.config1()
+    out["c1_A"] = A
+    out["c1_truth"] = _c(O.hafnian_sieve(A, precision="dd"))
+    out["c1_ref64"] = _c(O.hafnian_sieve(A, precision="double"))
+    mats = W.config2(count=8, ones=1)
+    for i, M in enumerate(mats):
+        out["c2_A_%d" % i] = M
+        out["c2_truth_%d" % i] = _c(O.hafnian_sieve(M, precision="dd"))
+        out["c2_ref64_%d" % i] = _c(O.hafnian_sieve(M, precision="double"))
+    out["c2_count"] = np.int64(len(mats))
+    reqs = W.config4(count=32)
+    for i, (A, rpt, mu) in enumerate(reqs):
+        out["c4_A_%d" % i] = A
+        out["c4_rpt_%d" % i] = np.asarray(rpt, dtype=np.int64)
+        out["c4_mu_%d" % i] = mu
+        out["c4_truth_%d" % i] = _c(O.kan(A, rpt, mu=mu, loop=True))
+        out["c4_ref64_%d" % i] = _c(O.hafnian_sieve(A, loop=True, rpt=rpt, mu=mu, precision="double"))
+    out["c4_count"] = np.int64(len(reqs))
+    stream = [r for r in W.config5(count=200) if r[0].shape[0] <= 28][:40]
+    for i, (A, loop) in enumerate(stream):
+        out["c5_A_%d" % i] = A
+        out["c5_loop_%d" % i] = np.bool_(loop)
+        out["c5_truth_%d" % i] = _c(O.hafnian_sieve(A, loop=loop, precision="dd"))
+        out["c5_ref64_%d" % i] = _c(O.hafnian_sieve(A, loop=loop, precision="double"))
+    out["c5_count"] = np.int64(len(stream))
+    np.savez_compressed(os.path.join(HERE, "golden_small.npz"), **out)
+
+
+def n40():
+    out = {}
+    t0 = time.time()
+    A = W.config3()
+    out["c3_A"] = A
+    out["c3_ref64"] = _c(O.hafnian_sieve(A, loop=True, precision="double"))
+    print("config3 ref64 %.0fs" % (time.time() - t0), flush=True)
+    out["c3_truth"] = _c(O.hafnian_sieve(A, loop=True, precision="dd", variant=1))
+    print("config3 dd %.0fs" % (time.time() - t0), flush=True)
+    for loop in (False, True):
+        S, t = structured(20, loop, 40 + loop)
+        key = "s40_%s" % ("loop" if loop else "plain")
+        out[key + "_A"] = S
+        out[key + "_truth"] = t
+        out[key + "_ref64"] = _c(O.hafnian_sieve(S, loop=loop, precision="double"))
+        print(key, "%.0fs" % (time.time() - t0), flush=True)
+    np.savez_compressed(os.path.join(HERE, "golden_n40.npz"), **out)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "small"
+    if what in ("small", "all"):
+        small()
+    if what in ("n40", "all"):
+        n40()

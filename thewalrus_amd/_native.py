@@ -44,6 +44,8 @@ EXPORTS = [
     "wh_plan_create",
     "wh_plan_run",
     "wh_plan_info",
+    "wh_plan_partials",
+    "wh_plan_chunks",
     "wh_plan_destroy",
 ]
 
@@ -75,6 +77,8 @@ def load(path=None):
         L.wh_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.wh_plan_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
         L.wh_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.wh_plan_partials.argtypes = [ctypes.c_void_p, _dp]
+        L.wh_plan_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         for name in EXPORTS:
             getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
         if path is None:

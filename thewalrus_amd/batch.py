@@ -56,9 +56,11 @@ class DeviceBatch:
         device (int): HIP device index
         chunk_terms (int): terms per workgroup (0 = automatic)
         force_generic (bool): run every request on the general kernel
+        full_products (bool): on the all-ones kernel, form every power as the reference does
+            (h - 1 products per term) instead of the half-products formulation
     """
 
-    def __init__(self, preps, device=0, chunk_terms=0, force_generic=False):
+    def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False):
         import torch
 
         self.torch = torch
@@ -83,7 +85,7 @@ class DeviceBatch:
                 raise ValueError("request %d: %d edges exceed the device limit %d" % (i, E, _native.MAX_EDGES))
             r.E = E
             r.glynn = int(bool(p.glynn))
-            r.options = 1 if force_generic else 0
+            r.options = (1 if force_generic else 0) | (2 if full_products else 0)
             for k in range(E):
                 r.edge_reps[k] = int(p.edge_reps[k])
             r.a_off = off

@@ -154,21 +154,22 @@ enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1 };
 constexpr int kPm1MaxT = 6;
 
 struct GroupKey {
-    int kind, T, real, loop;
+    int kind, T, real, loop, half;
     bool operator<(const GroupKey& o) const {
         if (kind != o.kind) return kind < o.kind;
         if (T != o.T) return T < o.T;
         if (real != o.real) return real < o.real;
-        return loop < o.loop;
+        if (loop != o.loop) return loop < o.loop;
+        return half < o.half;
     }
     bool operator==(const GroupKey& o) const {
-        return kind == o.kind && T == o.T && real == o.real && loop == o.loop;
+        return kind == o.kind && T == o.T && real == o.real && loop == o.loop && half == o.half;
     }
 };
 
-GroupKey classify(const wh_job_dev& j, bool force_generic) {
-    GroupKey g{KIND_GENERIC, 0, 0, 0};
-    if (force_generic) return g;
+GroupKey classify(const wh_job_dev& j, int options) {
+    GroupKey g{KIND_GENERIC, 0, 0, 0, 0};
+    if (options & WH_OPT_GENERIC) return g;
     bool all_one = true;
     for (int i = 0; i < j.E; ++i) all_one = all_one && j.e[i] == 1;
     bool glynn = (j.flags & WH_F_GLYNN) != 0, odd = (j.flags & WH_F_ODD) != 0;
@@ -178,6 +179,7 @@ GroupKey classify(const wh_job_dev& j, bool force_generic) {
         g.T = T;
         g.real = (j.flags & WH_F_REAL) ? 1 : 0;
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
+        g.half = (options & WH_OPT_FULL) ? 0 : 1;
     }
     return g;
 }
@@ -214,11 +216,13 @@ int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
 
 double term_flops(const wh_job_dev& j, const GroupKey& g) {
     // algorithmic FP64 flops of the products per term, as the kernels run them:
-    // PM1: h - 1 products of n x n (8 flops per complex multiply-add, 2 per real one);
+    // PM1: h - 1 products of n x n (8 flops per complex multiply-add, 2 per real one), or
+    // ceil(h/2) - 1 with the half-products formulation;
     // GENERIC: (min(h + 1, m) - 2) products of m x m with m = 2 * nnz(delta) <= 2E (upper bound)
     int h = j.N / 2;
     int m = 2 * j.E;
     int np = std::min(h + 1, m) - 2;
+    if (g.kind == KIND_PM1 && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
     double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;
     return c * m * (double)m * m * np;
@@ -247,7 +251,7 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
         int rc = describe(reqs[i], tmp[i]);
         if (rc) return rc;
         tmp[i].out_index = i;
-        keys[i] = classify(tmp[i], (reqs[i].options & WH_OPT_GENERIC) != 0);
+        keys[i] = classify(tmp[i], reqs[i].options);
         if (keys[i].kind == KIND_GENERIC && 2 * reqs[i].E > kGenericMaxSide)
             return fail(WH_EUNSUPPORTED, "matrix side %d exceeds the generic kernel limit %d",
                         2 * reqs[i].E, kGenericMaxSide);
@@ -296,7 +300,8 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
         int64_t nb = g.block_end - g.block_begin;
         hipError_t e;
         if (g.key.kind == KIND_PM1)
-            e = wh_launch_pm1(g.key.T, g.key.real != 0, g.key.loop != 0, gj, nj, pool, partials, nb,
+            e = wh_launch_pm1(g.key.T, g.key.half != 0, g.key.real != 0, g.key.loop != 0, gj, nj, pool,
+                              partials, nb,
                               g.block_begin, g.nw, s);
         else
             e = wh_launch_generic(gj, nj, pool, partials, nb, g.block_begin, g.MM, s);
@@ -360,13 +365,7 @@ int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_pl
 
 int wh_plan_run(wh_plan* p, const double* dpool, double* dout, void* stream) {
     if (!p || !dpool || !dout) return fail(WH_EINVAL, "null plan or buffer");
-    hipStream_t s = (hipStream_t)stream;
-    if (!s) {
-        Context* c;
-        int rc = ensure_context(p->device, &c);
-        if (rc) return rc;
-        s = c->stream;
-    }
+    hipStream_t s = (hipStream_t)stream;  // NULL: the device's default stream
     HIP_TRY(run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s));
     return WH_OK;
 }
@@ -376,6 +375,26 @@ int wh_plan_info(const wh_plan* p, int64_t* terms, int64_t* blocks, double* flop
     if (terms) *terms = p->layout.terms;
     if (blocks) *blocks = p->layout.blocks;
     if (flops) *flops = p->layout.flops;
+    return WH_OK;
+}
+
+int wh_plan_partials(const wh_plan* p, double* host_out) {
+    if (!p || !host_out) return fail(WH_EINVAL, "null plan or buffer");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(host_out, p->d_partials, p->layout.blocks * sizeof(double4), hipMemcpyDeviceToHost));
+    // reorder nothing: slot b belongs to the job whose chunk range holds b (request order is
+    // recovered by the caller through wh_plan_chunks)
+    return WH_OK;
+}
+
+int wh_plan_chunks(const wh_plan* p, int64_t* first_chunk, int64_t* nchunks, int64_t* chunk_terms) {
+    if (!p) return fail(WH_EINVAL, "null plan");
+    for (const wh_job_dev& j : p->layout.jobs) {
+        int64_t i = j.out_index;
+        if (first_chunk) first_chunk[i] = j.chunk_base;
+        if (nchunks) nchunks[i] = j.nchunks;
+        if (chunk_terms) chunk_terms[i] = j.chunk_terms;
+    }
     return WH_OK;
 }
 

@@ -52,7 +52,120 @@ __device__ __forceinline__ double col_sign(uint64_t t, int c, int h) {
     return ((t >> (h - 1 - i)) & 1ull) ? 1.0 : -1.0;
 }
 
-template <int T, bool REAL, bool LOOP>
+
+// ---- tile helpers: lane (ti, tj) owns rows I0..I0+T-1, columns J0..J0+T-1 of the power
+
+template <int T, class E>
+__device__ __forceinline__ void tile_store(E* Pw, const E (&acc)[T][T], int I0, int J0) {
+#pragma unroll
+    for (int r = 0; r < T; ++r)
+#pragma unroll
+        for (int c = 0; c < T; ++c) Pw[(I0 + r) * (8 * T) + J0 + c] = acc[r][c];
+}
+
+// acc = (Pw Ms) diag(sign_J): one step of the power sequence, k summed in order
+template <int T, class E>
+__device__ __forceinline__ void tile_product(const E* Pw, const E* Ms, E (&acc)[T][T], int I0, int J0,
+                                             int n, const double (&sgnJ)[T]) {
+    constexpr int S = 8 * T;
+    E nacc[T][T];
+#pragma unroll
+    for (int r = 0; r < T; ++r)
+#pragma unroll
+        for (int c = 0; c < T; ++c) ezero(nacc[r][c]);
+#pragma unroll 2
+    for (int k = 0; k < n; ++k) {
+        E av[T], bv[T];
+#pragma unroll
+        for (int r = 0; r < T; ++r) av[r] = Pw[(I0 + r) * S + k];
+#pragma unroll
+        for (int c = 0; c < T; ++c) bv[c] = Ms[k * S + J0 + c];
+#pragma unroll
+        for (int r = 0; r < T; ++r)
+#pragma unroll
+            for (int c = 0; c < T; ++c) emac(nacc[r][c], av[r], bv[c]);
+    }
+#pragma unroll
+    for (int r = 0; r < T; ++r)
+#pragma unroll
+        for (int c = 0; c < T; ++c) acc[r][c] = escale(nacc[r][c], sgnJ[c]);
+}
+
+template <int T, class E>
+__device__ __forceinline__ E tile_trace(const E (&acc)[T][T], bool diag) {
+    E tr;
+    ezero(tr);
+    if (diag) {
+#pragma unroll
+        for (int r = 0; r < T; ++r) tr = eadd(tr, acc[r][r]);
+    }
+    return tr;
+}
+
+// lane part of <acc, X^T> = sum_(i in I, j in J) acc[i][j] X[j][i], X in LDS
+template <int T, class E>
+__device__ __forceinline__ E tile_tdot(const E* X, const E (&acc)[T][T], int I0, int J0) {
+    E s;
+    ezero(s);
+#pragma unroll
+    for (int c = 0; c < T; ++c)
+#pragma unroll
+        for (int r = 0; r < T; ++r) emac(s, acc[r][c], X[(J0 + c) * (8 * T) + I0 + r]);
+    return s;
+}
+
+// lane part of XD_d B D
+template <int T, class E>
+__device__ __forceinline__ E tile_bilinear(const E (&acc)[T][T], const E (&xdI)[T], const E (&dJ)[T]) {
+    E bl;
+    ezero(bl);
+#pragma unroll
+    for (int r = 0; r < T; ++r) {
+        E rowsum;
+        ezero(rowsum);
+#pragma unroll
+        for (int c = 0; c < T; ++c) emac(rowsum, acc[r][c], dJ[c]);
+        emac(bl, xdI[r], rowsum);
+    }
+    return bl;
+}
+
+__device__ __forceinline__ double shfl_e(double v, int src) { return __shfl(v, src, WH_WAVE); }
+__device__ __forceinline__ cplx shfl_e(cplx v, int src) {
+    return cmk(__shfl(v.re, src, WH_WAVE), __shfl(v.im, src, WH_WAVE));
+}
+__device__ __forceinline__ double shfl_xor_e(double v, int m) { return __shfl_xor(v, m, WH_WAVE); }
+__device__ __forceinline__ cplx shfl_xor_e(cplx v, int m) {
+    return cmk(__shfl_xor(v.re, m, WH_WAVE), __shfl_xor(v.im, m, WH_WAVE));
+}
+
+// (B D)[I0 + r]: rows summed over the 8 column tiles of the lane's row group
+template <int T, class E>
+__device__ __forceinline__ void tile_zvec(const E (&acc)[T][T], const E (&dJ)[T], E (&z)[T]) {
+#pragma unroll
+    for (int r = 0; r < T; ++r) {
+        ezero(z[r]);
+#pragma unroll
+        for (int c = 0; c < T; ++c) emac(z[r], acc[r][c], dJ[c]);
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) z[r] = eadd(z[r], shfl_xor_e(z[r], m));
+    }
+}
+
+// (XD_d B)[J0 + c]: columns summed over the 8 row tiles of the lane's column group
+template <int T, class E>
+__device__ __forceinline__ void tile_yvec(const E (&acc)[T][T], const E (&xdI)[T], E (&y)[T]) {
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+        ezero(y[c]);
+#pragma unroll
+        for (int r = 0; r < T; ++r) emac(y[c], xdI[r], acc[r][c]);
+#pragma unroll
+        for (int m = 8; m < 64; m <<= 1) y[c] = eadd(y[c], shfl_xor_e(y[c], m));
+    }
+}
+
+template <int T, bool REAL, bool LOOP, bool HALF>
 __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __restrict__ pool,
                                          double4* __restrict__ partials, int64_t b,
                                          unsigned char* smem) {
@@ -140,61 +253,78 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             for (int c = 0; c < T; ++c) dJ[c] = Dv[J0 + c];
         }
 
-        for (int a = 1; a <= h; ++a) {
-            // trace of P_a (diagonal tiles) and, for the loop, XD_d P_(a-1) D (a >= 2 uses
-            // the power held in acc one step earlier; v_1 = XD_d . D is taken below)
-            E tr;
-            ezero(tr);
-            if (ti == tj) {
-#pragma unroll
-                for (int r = 0; r < T; ++r) tr = eadd(tr, acc[r][r]);
+        if constexpr (!HALF) {
+            // the reference's order: B^(a+1) = B^a B for a < h, p_a = tr(B^a)
+            for (int a = 1; a <= h; ++a) {
+                E tr = tile_trace<T>(acc, ti == tj);
+                E bl;
+                ezero(bl);
+                if (LOOP && a <= h - 1) bl = tile_bilinear<T>(acc, xdI, dJ);
+                cplx trc = wave_csum(etoc(tr));
+                cplx blc = LOOP ? wave_csum(etoc(bl)) : cmk(0.0, 0.0);
+                if (lane == 0) {
+                    pt[a] = trc;
+                    if (LOOP && a <= h - 1) lv[a + 1] = blc;
+                }
+                if (a == h) break;
+                tile_store<T>(Pw, acc, I0, J0);
+                tile_product<T>(Pw, Ms, acc, I0, J0, n, sgnJ);
             }
-            E bl;
-            ezero(bl);
-            if (LOOP && a <= h - 1) {
+        } else {
+            // powers up to B^A, A = ceil(h/2); tr(B^(2a)) = <B^a, (B^a)^T> and
+            // tr(B^(2a+1)) = <B^(a+1), (B^a)^T>.  Loop scalars v_k = XD B^(k-1) D from
+            // y_a = XD B^a (column tiles) and z_a = B^a D (row tiles):
+            // v_(2a) = y_a . z_(a-1), v_(2a+1) = y_a . z_a, dotted on the diagonal lanes, where
+            // the lane's column and row tiles cover the same indices.
+            const int A = (h + 1) / 2;
+            E zprev[T];
+            if (LOOP) {
 #pragma unroll
-                for (int r = 0; r < T; ++r) {
-                    E rowsum;
-                    ezero(rowsum);
+                for (int r = 0; r < T; ++r) zprev[r] = Dv[I0 + r];  // z_0 = D
+            }
+            tile_store<T>(Pw, acc, I0, J0);
+            for (int a = 1; a <= A; ++a) {
+                E qo, qe;
+                if (a == 1) {
+                    qo = tile_trace<T>(acc, ti == tj);   // tr(B)
+                    qe = tile_tdot<T>(Pw, acc, I0, J0);  // <B, B^T>
+                } else {
+                    tile_product<T>(Pw, Ms, acc, I0, J0, n, sgnJ);  // acc = B^a
+                    qo = tile_tdot<T>(Pw, acc, I0, J0);            // Pw holds B^(a-1)
+                    tile_store<T>(Pw, acc, I0, J0);
+                    qe = tile_tdot<T>(Pw, acc, I0, J0);
+                }
+                cplx qoc = wave_csum(etoc(qo));
+                cplx qec = wave_csum(etoc(qe));
+                const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+                if (lane == 0) {
+                    pt[ko] = qoc;
+                    pt[ke] = qec;  // ke <= h + 1 < WH_EMAX + 2
+                }
+                if (LOOP) {
+                    E y[T], z[T];
+                    tile_yvec<T>(acc, xdI, y);
+                    tile_zvec<T>(acc, dJ, z);
+                    E de, dodd;
+                    ezero(de);
+                    ezero(dodd);
+                    if (ti == tj) {
 #pragma unroll
-                    for (int c = 0; c < T; ++c) emac(rowsum, acc[r][c], dJ[c]);
-                    emac(bl, xdI[r], rowsum);
+                        for (int c = 0; c < T; ++c) {
+                            emac(de, y[c], zprev[c]);
+                            emac(dodd, y[c], z[c]);
+                        }
+                    }
+                    cplx vec_ = wave_csum(etoc(de));
+                    cplx voc_ = wave_csum(etoc(dodd));
+                    if (lane == 0) {
+                        if (2 * a <= h) lv[2 * a] = vec_;
+                        if (2 * a + 1 <= h) lv[2 * a + 1] = voc_;
+                    }
+#pragma unroll
+                    for (int r = 0; r < T; ++r) zprev[r] = z[r];
                 }
             }
-            cplx trc = etoc(tr), blc = etoc(bl);
-            trc = wave_csum(trc);
-            if (LOOP) blc = wave_csum(blc);
-            if (lane == 0) {
-                pt[a] = trc;
-                if (LOOP && a <= h - 1) lv[a + 1] = blc;
-            }
-            if (a == h) break;
-            // P_(a+1) = (P_a M) diag(d)
-#pragma unroll
-            for (int r = 0; r < T; ++r)
-#pragma unroll
-                for (int c = 0; c < T; ++c) Pw[(I0 + r) * S + J0 + c] = acc[r][c];
-            E nacc[T][T];
-#pragma unroll
-            for (int r = 0; r < T; ++r)
-#pragma unroll
-                for (int c = 0; c < T; ++c) ezero(nacc[r][c]);
-#pragma unroll 2
-            for (int k = 0; k < n; ++k) {
-                E av[T], bv[T];
-#pragma unroll
-                for (int r = 0; r < T; ++r) av[r] = Pw[(I0 + r) * S + k];
-#pragma unroll
-                for (int c = 0; c < T; ++c) bv[c] = Ms[k * S + J0 + c];
-#pragma unroll
-                for (int r = 0; r < T; ++r)
-#pragma unroll
-                    for (int c = 0; c < T; ++c) emac(nacc[r][c], av[r], bv[c]);
-            }
-#pragma unroll
-            for (int r = 0; r < T; ++r)
-#pragma unroll
-                for (int c = 0; c < T; ++c) acc[r][c] = escale(nacc[r][c], sgnJ[c]);
         }
         if (LOOP) {
             // v_1 = XD_d . D
@@ -245,7 +375,7 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
 
 }  // namespace
 
-template <int T, bool REAL, bool LOOP>
+template <int T, bool REAL, bool LOOP, bool HALF>
 __global__ void __launch_bounds__(256)
 wh_sieve_pm1(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
              double4* __restrict__ partials, int64_t block_offset) {
@@ -256,7 +386,7 @@ wh_sieve_pm1(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restr
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
     }
-    pm1_body<T, REAL, LOOP>(jobs[lo], pool, partials, b, wh_smem);
+    pm1_body<T, REAL, LOOP, HALF>(jobs[lo], pool, partials, b, wh_smem);
 }
 
 size_t wh_pm1_lds_bytes(int T, bool real, int nw) {
@@ -266,39 +396,47 @@ size_t wh_pm1_lds_bytes(int T, bool real, int nw) {
            nw * sizeof(double4);
 }
 
-template <int T, bool REAL, bool LOOP>
+template <int T, bool REAL, bool LOOP, bool HALF>
 static hipError_t launch_t(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
                            int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
     size_t lds = wh_pm1_lds_bytes(T, REAL, nw);
-    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1<T, REAL, LOOP>,
+    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1<T, REAL, LOOP, HALF>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((wh_sieve_pm1<T, REAL, LOOP>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds,
+    hipLaunchKernelGGL((wh_sieve_pm1<T, REAL, LOOP, HALF>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds,
                        stream, jobs, njobs, (const cplx*)pool, (double4*)partials, block_offset);
     return hipGetLastError();
 }
 
-template <int T>
+template <int T, bool HALF>
 static hipError_t launch_rl(bool real, bool loop, const wh_job_dev* jobs, int njobs, const void* pool,
                             void* partials, int64_t nblocks, int64_t off, int nw, hipStream_t s) {
     if (real) {
-        if (loop) return launch_t<T, true, true>(jobs, njobs, pool, partials, nblocks, off, nw, s);
-        return launch_t<T, true, false>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+        if (loop) return launch_t<T, true, true, HALF>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+        return launch_t<T, true, false, HALF>(jobs, njobs, pool, partials, nblocks, off, nw, s);
     }
-    if (loop) return launch_t<T, false, true>(jobs, njobs, pool, partials, nblocks, off, nw, s);
-    return launch_t<T, false, false>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    if (loop) return launch_t<T, false, true, HALF>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    return launch_t<T, false, false, HALF>(jobs, njobs, pool, partials, nblocks, off, nw, s);
 }
 
-hipError_t wh_launch_pm1(int T, bool real, bool loop, const wh_job_dev* jobs, int njobs,
+template <int T>
+static hipError_t launch_h(bool half, bool real, bool loop, const wh_job_dev* jobs, int njobs,
+                           const void* pool, void* partials, int64_t nblocks, int64_t off, int nw,
+                           hipStream_t s) {
+    if (half) return launch_rl<T, true>(real, loop, jobs, njobs, pool, partials, nblocks, off, nw, s);
+    return launch_rl<T, false>(real, loop, jobs, njobs, pool, partials, nblocks, off, nw, s);
+}
+
+hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_dev* jobs, int njobs,
                          const void* pool, void* partials, int64_t nblocks, int64_t block_offset,
                          int nw, hipStream_t stream) {
     switch (T) {
-        case 1: return launch_rl<1>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
-        case 2: return launch_rl<2>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
-        case 3: return launch_rl<3>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
-        case 4: return launch_rl<4>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
-        case 5: return launch_rl<5>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
-        case 6: return launch_rl<6>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 1: return launch_h<1>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 2: return launch_h<2>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 3: return launch_h<3>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 4: return launch_h<4>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 5: return launch_h<5>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+        case 6: return launch_h<6>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
         default: return hipErrorInvalidValue;
     }
 }

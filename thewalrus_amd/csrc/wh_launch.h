@@ -10,7 +10,7 @@ size_t wh_generic_lds_bytes(int MM);
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
                              int64_t nblocks, int64_t block_offset, int MM, hipStream_t stream);
 size_t wh_pm1_lds_bytes(int T, bool real, int nw);
-hipError_t wh_launch_pm1(int T, bool real, bool loop, const wh_job_dev* jobs, int njobs,
+hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_dev* jobs, int njobs,
                          const void* pool, void* partials, int64_t nblocks, int64_t block_offset,
                          int nw, hipStream_t stream);
 hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partials, void* out,
