@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=128)
     ap.add_argument("--workload", default="stream", choices=["stream", "batch24"])
+    ap.add_argument("--nmin", type=int, default=8, help="drop stream requests with n < nmin")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percall", action="store_true")
     args = ap.parse_args()
@@ -118,8 +119,10 @@ def main():
 
     torch.cuda.init()
     if args.workload == "stream":
-        reqs = stream_requests(args.requests)
+        reqs = [r for r in stream_requests(args.requests) if r[0].shape[0] >= args.nmin]
         wl = "mixed stream: even n uniform in [8,40], 50% loop, complex128 CN(0,1/n) symmetrised, seed 141"
+        if args.nmin > 8:
+            wl += " (only n >= %d)" % args.nmin
     else:
         reqs = batch24_requests()
         wl = "4096 real symmetric 24x24, U[-1,1] symmetrised, seed 138"

@@ -134,10 +134,10 @@ __device__ __forceinline__ double shfl_e(double v, int src) { return __shfl(v, s
 __device__ __forceinline__ cplx shfl_e(cplx v, int src) {
     return cmk(__shfl(v.re, src, WH_WAVE), __shfl(v.im, src, WH_WAVE));
 }
-__device__ __forceinline__ double shfl_xor_e(double v, int m) { return __shfl_xor(v, m, WH_WAVE); }
-__device__ __forceinline__ cplx shfl_xor_e(cplx v, int m) {
-    return cmk(__shfl_xor(v.re, m, WH_WAVE), __shfl_xor(v.im, m, WH_WAVE));
-}
+__device__ __forceinline__ double group8_e(double v) { return group8_sum(v); }
+__device__ __forceinline__ cplx group8_e(cplx v) { return cmk(group8_sum(v.re), group8_sum(v.im)); }
+__device__ __forceinline__ double stride8_e(double v) { return stride8_sum(v); }
+__device__ __forceinline__ cplx stride8_e(cplx v) { return cmk(stride8_sum(v.re), stride8_sum(v.im)); }
 
 // (B D)[I0 + r]: rows summed over the 8 column tiles of the lane's row group
 template <int T, class E>
@@ -147,8 +147,7 @@ __device__ __forceinline__ void tile_zvec(const E (&acc)[T][T], const E (&dJ)[T]
         ezero(z[r]);
 #pragma unroll
         for (int c = 0; c < T; ++c) emac(z[r], acc[r][c], dJ[c]);
-#pragma unroll
-        for (int m = 1; m < 8; m <<= 1) z[r] = eadd(z[r], shfl_xor_e(z[r], m));
+        z[r] = group8_e(z[r]);
     }
 }
 
@@ -160,8 +159,7 @@ __device__ __forceinline__ void tile_yvec(const E (&acc)[T][T], const E (&xdI)[T
         ezero(y[c]);
 #pragma unroll
         for (int r = 0; r < T; ++r) emac(y[c], xdI[r], acc[r][c]);
-#pragma unroll
-        for (int m = 8; m < 64; m <<= 1) y[c] = eadd(y[c], shfl_xor_e(y[c], m));
+        y[c] = stride8_e(y[c]);
     }
 }
 

@@ -77,14 +77,61 @@ struct csum {
     }
 };
 
-__device__ __forceinline__ double shfl_xor_d(double v, int mask) {
-    return __shfl_xor(v, mask, WH_WAVE);
+// ---- cross-lane sums on the VALU (DPP within 16-lane rows, permlane swaps across rows).
+// Every step adds a lane's value to its partner's; partners add the same two numbers, so all
+// lanes of a group end with bit-identical sums.  Callers keep the whole wave active.
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
-// Butterfly sum over the whole wave; every lane receives the total.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 1; m < WH_WAVE; m <<= 1) v += shfl_xor_d(v, m);
+#define WH_DPP_XOR1 0xB1        // quad_perm [1,0,3,2]
+#define WH_DPP_XOR2 0x4E        // quad_perm [2,3,0,1]
+#define WH_DPP_HALF_MIRROR 0x141 // lane i <-> 7 - i within 8
+#define WH_DPP_MIRROR 0x140     // lane i <-> 15 - i within 16
+#define WH_DPP_ROR8 0x128       // lane i <-> i ^ 8 within 16
+
+// v + the value of lane ^ 16 (rows 0<->1, 2<->3)
+__device__ __forceinline__ double swap16_sum(double v) {
+    unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    double x = __hiloint2double((int)b[0], (int)a[0]);
+    double y = __hiloint2double((int)b[1], (int)a[1]);
+    return x + y;
+}
+// v + the value of lane ^ 32
+__device__ __forceinline__ double swap32_sum(double v) {
+    unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    double x = __hiloint2double((int)b[0], (int)a[0]);
+    double y = __hiloint2double((int)b[1], (int)a[1]);
+    return x + y;
+}
+
+// sum over the 8 lanes with the same lane / 8 (lanes 8g .. 8g + 7)
+__device__ __forceinline__ double group8_sum(double v) {
+    v += dpp_d<WH_DPP_XOR1>(v);
+    v += dpp_d<WH_DPP_XOR2>(v);
+    v += dpp_d<WH_DPP_HALF_MIRROR>(v);
     return v;
+}
+// sum over the 8 lanes with the same lane % 8 (lanes c, c + 8, ..., c + 56)
+__device__ __forceinline__ double stride8_sum(double v) {
+    v += dpp_d<WH_DPP_ROR8>(v);
+    v = swap16_sum(v);
+    return swap32_sum(v);
+}
+// Sum over the whole wave; every lane receives the total.
+__device__ __forceinline__ double wave_sum(double v) {
+    v = group8_sum(v);
+    v += dpp_d<WH_DPP_MIRROR>(v);
+    v = swap16_sum(v);
+    return swap32_sum(v);
 }
 __device__ __forceinline__ cplx wave_csum(cplx z) { return cmk(wave_sum(z.re), wave_sum(z.im)); }
 
