@@ -1,16 +1,24 @@
 """Benchmark: hafnians/s on the mixed n in [8, 40] complex128 request stream, one MI355X.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests R] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests R]
 
-A step is one pass of the sieve over one batch of R requests of the stream (even n uniform in
-[8, 40], half loop hafnians, entries complex N(0, 1/n) symmetrised, seed 141), with every
-request's reordered matrix already resident in HBM.  ``value`` = R * K / (time of K steps).
-Printed beside it, in the same JSON line:
+Workload (the metric's): the serving stream of the benchmark configurations -- even n
+uniform in [8, 40], half of them loop hafnians, entries complex N(0, 1/n) symmetrised
+(``thewalrus_amd.workloads.config5``, seed 141).  A step is one pass of the sieve over a batch
+of R requests of that stream with every request's reordered matrix already resident in HBM;
+``value`` = R * K / (wall time of K steps).  The same JSON line also carries:
   * per_call: the same requests through the drop-in ``hafnian()`` one call at a time
-    (synchronous, host buffers) and the batched/per-call throughput ratio;
-  * roofline: FP64 products of the dominant kernel (algorithmic flops) / its HIP-event time;
-  * cpu_baseline: the CPU oracle (C++ restatement of the reference algorithm, all host cores)
-    on a bounded sample of the same stream, scaled to hafnians/s by algorithmic work.
+    (synchronous, host buffers), and the batched / per-call throughput ratio;
+  * serving: the continuous-batching server fed by Poisson arrivals at 0.5x and 0.9x of the
+    measured batched capacity: p50 / p99 latency (submit to result) and throughput;
+  * small_stream: batched vs per-call on the same stream restricted to n <= 24, where
+    per-call dispatch cost matters;
+  * configs: the other benchmark configurations (1: n = 20 per call; 2: 4096 real 24 x 24 in
+    one launch; 3: n = 40 loop single solve; 4: 1024 hafnian_repeated requests);
+  * roofline: FP64 product flops of the dominant kernel / its HIP-event time (same stream);
+  * cpu_baseline: the CPU oracle (C++ restatement of the reference algorithm, OpenMP over
+    terms, all host cores) on a bounded sample of the stream, scaled to hafnians/s by
+    algorithmic work.
 """
 import argparse
 import json
@@ -23,37 +31,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from thewalrus_amd import workloads as W  # noqa: E402
+
 FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
-
-
-def stream_requests(R, seed=141):
-    """The mixed request stream: (A, loop) pairs."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(R):
-        n = 2 * int(rng.integers(4, 21))
-        loop = bool(rng.random() < 0.5)
-        s = np.sqrt(1.0 / (2 * n))
-        B = rng.normal(scale=s, size=(n, n)) + 1j * rng.normal(scale=s, size=(n, n))
-        out.append(((B + B.T) / 2, loop))
-    return out
-
-
-def batch24_requests(R=4096, seed=138):
-    rng = np.random.default_rng(seed)
-    M = rng.uniform(-1, 1, size=(R, 24, 24))
-    return [((m + m.T) / 2, False) for m in M]
-
-
-def w_glynn(n, real=False):
-    """Algorithmic flops of the reference's products for one even-n hafnian, all rpt = 1."""
-    h = n // 2
-    return 2 ** (h - 1) * (2.0 if real else 8.0) * n ** 3 * max(h - 1, 0)
+METRIC = "hafnians/s + p50/p99 ms, mixed n in [8,40] c128 stream vs per-call GPU"
 
 
 def timed_runs(batch, k, torch, s):
-    """Run the plan k times on stream s between two synchronisations; HIP-event time of the
-    kernels on s and wall time (s)."""
+    """Run a DeviceBatch k times on stream s between synchronisations: (HIP-event s, wall s)."""
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -66,34 +51,103 @@ def timed_runs(batch, k, torch, s):
     return e0.elapsed_time(e1) / 1e3, wall
 
 
+def per_call(reqs, tw):
+    t0 = time.perf_counter()
+    vals = [tw.hafnian(A, loop=l) for A, l in reqs]
+    return len(reqs) / (time.perf_counter() - t0), vals
+
+
+def serve_run(reqs, qps, Server):
+    arrivals = W.arrivals(len(reqs), qps)
+    with Server(slots=4, slot_pool_mb=16) as srv:
+        t0 = time.perf_counter()
+        for (A, l), ta in zip(reqs, arrivals):
+            wait = ta - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(A, loop=l)
+        srv.drain()
+        s = srv.stats.summary()
+    s["offered_qps"] = qps
+    return s
+
+
 def cpu_baseline(reqs, budget_s=20.0):
-    """Oracle (C++ restatement, OpenMP over terms) on a sample of the stream; hafnians/s of
-    the whole stream estimated from the measured algorithmic flop rate."""
+    """Oracle on a sample of the stream (largest n <= 28 first); hafnians/s of the whole
+    stream from the measured algorithmic flop rate."""
     from oracle import oracle as O
 
     cores = len(os.sched_getaffinity(0))
     sample = sorted([r for r in reqs if r[0].shape[0] <= 28], key=lambda r: -r[0].shape[0])
     done_w, spent, used = 0.0, 0.0, []
-    O.hafnian_sieve(np.ones((8, 8)), nthreads=cores)  # load + warm
+    O.hafnian_sieve(np.ones((8, 8)), nthreads=cores)
     for A, loop in sample:
         if spent > budget_s:
             break
         t0 = time.perf_counter()
         O.hafnian_sieve(A, loop=loop, nthreads=cores)
         spent += time.perf_counter() - t0
-        done_w += w_glynn(A.shape[0])
+        done_w += W.w_glynn(A.shape[0])
         used.append(A.shape[0])
     rate = done_w / spent
-    total_w = sum(w_glynn(A.shape[0]) for A, _ in reqs)
+    total_w = sum(W.w_glynn(A.shape[0]) for A, _ in reqs)
     return {
         "value": len(reqs) / (total_w / rate),
         "unit": "hafnians/s",
         "cores": cores,
         "kind": "port",
         "sample": "CPU oracle (C++ restatement of thewalrus _calc_hafnian/_calc_loop_hafnian, "
-        "OpenMP over terms) on %d stream requests with n in [%d, %d] (%.1f s, %.3g flop/s), "
-        "scaled to the full stream by algorithmic product flops" % (len(used), min(used), max(used), spent, rate),
+        "OpenMP over terms, %d threads) on %d stream requests with n in [%d, %d] (%.1f s, %.3g flop/s "
+        "of the reference algorithm's products), scaled to the full stream by those flops"
+        % (cores, len(used), min(used), max(used), spent, rate),
     }
+
+
+def configs_run(torch, tw, B, stream):
+    """Configurations 1-4, one measurement each (inputs resident for the batched ones)."""
+    out = {}
+    A1 = W.config1()
+    tw.hafnian(A1)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        tw.hafnian(A1)
+    out["config1_n20_per_call_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+
+    mats = W.config2()
+    b2 = B.DeviceBatch([B.prepare(M) for M in mats])
+    b2.run(stream)
+    ev, wall = timed_runs(b2, 3, torch, stream)
+    r2 = b2.results()
+    out["config2_real24_batch"] = {
+        "hafnians_per_s": 3 * len(mats) / wall,
+        "kernel_tflops": b2.flops / (ev / 3) / 1e12,
+        "all_ones_exact": bool(all(round(v.real) == 316234143225 for v in r2[-16:])),
+    }
+    b2.close()
+
+    A3 = W.config3()
+    tw.hafnian(A3, loop=True)
+    b3 = B.DeviceBatch([B.prepare(A3, loop=True)])
+    b3.run(stream)
+    ev, wall = timed_runs(b3, 3, torch, stream)
+    t0 = time.perf_counter()
+    tw.hafnian(A3, loop=True)
+    out["config3_n40_loop"] = {
+        "per_call_s": time.perf_counter() - t0,
+        "kernel_s": ev / 3,
+        "kernel_tflops": b3.flops / (ev / 3) / 1e12,
+        "frac_fp64_peak": b3.flops / (ev / 3) / FP64_PEAK,
+        "reference_algorithm_equivalent_tflops": W.w_glynn(40) / (ev / 3) / 1e12,
+    }
+    b3.close()
+
+    reqs4 = W.config4()
+    b4 = B.DeviceBatch([B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in reqs4])
+    b4.run(stream)
+    ev, wall = timed_runs(b4, 3, torch, stream)
+    out["config4_repeated_batch"] = {"requests_per_s": 3 * len(reqs4) / wall, "kernel_ms": ev / 3 * 1e3}
+    b4.close()
+    return out
 
 
 def main():
@@ -102,10 +156,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=128)
-    ap.add_argument("--workload", default="stream", choices=["stream", "batch24"])
+    ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=8, help="drop stream requests with n < nmin")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percall", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -116,20 +172,18 @@ def main():
 
     import thewalrus_amd as tw
     from thewalrus_amd import batch as B
+    from thewalrus_amd.serve import Server
 
     torch.cuda.init()
-    if args.workload == "stream":
-        reqs = [r for r in stream_requests(args.requests) if r[0].shape[0] >= args.nmin]
-        wl = "mixed stream: even n uniform in [8,40], 50% loop, complex128 CN(0,1/n) symmetrised, seed 141"
-        if args.nmin > 8:
-            wl += " (only n >= %d)" % args.nmin
-    else:
-        reqs = batch24_requests()
-        wl = "4096 real symmetric 24x24, U[-1,1] symmetrised, seed 138"
+    stream_all = W.config5(count=max(args.requests, args.serve_requests, 512))
+    reqs = [r for r in stream_all[: args.requests] if r[0].shape[0] >= args.nmin]
+    wl = "mixed stream: even n uniform in [8,40], 50% loop, complex128 CN(0,1/n) symmetrised, seed 141"
+    if args.nmin > 8:
+        wl += " (only n >= %d)" % args.nmin
     preps = [B.prepare(A, loop=l) for A, l in reqs]
     batch = B.DeviceBatch(preps)
+    stream = torch.cuda.Stream()  # default priority; kernels and events share it
 
-    stream = torch.cuda.Stream()  # default priority; the kernels and the events share it
     for _ in range(args.warmup):
         batch.run(stream)
     torch.cuda.synchronize()
@@ -137,22 +191,20 @@ def main():
     value = len(reqs) * args.steps / wall
     res = batch.results()
 
-    # dominant kernel: the largest tile class (n in [34, 40]), plain and loop separately
-    big = [i for i, (A, _) in enumerate(reqs) if A.shape[0] >= 34] if args.workload == "stream" else list(range(len(reqs)))
+    # dominant kernel: the largest tile class (n in [34, 40]); plain and loop launches timed
+    # separately on the stream they run on, the slower one reported
     roof = None
     best = None
     for want_loop in (False, True):
-        idx = [i for i in big if reqs[i][1] == want_loop]
+        idx = [i for i, (A, l) in enumerate(reqs) if A.shape[0] >= 34 and l == want_loop]
         if not idx:
             continue
         sub = B.DeviceBatch([preps[i] for i in idx])
         sub.run(stream)
-        torch.cuda.synchronize()
         t, _ = timed_runs(sub, 1, torch, stream)
-        fl = sub.flops
-        wref = sum(w_glynn(reqs[i][0].shape[0]) for i in idx)
+        wref = sum(W.w_glynn(reqs[i][0].shape[0]) for i in idx)
         if best is None or t > best[0]:
-            best = (t, fl, want_loop, len(idx), wref)
+            best = (t, sub.flops, want_loop, len(idx), wref)
         sub.close()
     if best:
         t, fl, lp, cnt, wref = best
@@ -163,7 +215,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": fl / t / FP64_PEAK,
             "traffic": None,
-            "kernel": "wh_sieve_pm1 T=5 %s (n in [34,40], %d requests)" % ("loop" if lp else "plain", cnt),
+            "kernel": "wh_sieve_pm1<5,complex,%s,half> (n in [34,40], %d requests)" % ("loop" if lp else "plain", cnt),
             "flops_formula": "sum over requests of 2^(h-1) * 8 n^3 * (ceil(h/2)-1): complex n x n "
             "products per term of the half-products formulation (powers up to B^ceil(h/2)); "
             "lower-order trace and loop terms not counted",
@@ -171,20 +223,38 @@ def main():
             "reference_algorithm_equivalent_tflops": wref / t / 1e12,
         }
 
-    per_call = None
+    pc = None
     if not args.no_percall:
-        t0 = time.perf_counter()
-        pc = [tw.hafnian(A, loop=l) for A, l in reqs]
-        dt = time.perf_counter() - t0
-        per_call = {
-            "value": len(reqs) / dt,
+        rate, vals = per_call(reqs, tw)
+        pc = {
+            "value": rate,
             "unit": "hafnians/s",
-            "max_rel_diff_vs_batched": float(max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(pc, res))),
+            "max_rel_diff_vs_batched": float(max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(vals, res))),
         }
 
+    serving = None
+    if not args.no_serve:
+        sreqs = stream_all[: args.serve_requests]
+        serving = {"requests": len(sreqs)}
+        for load in (0.5, 0.9):
+            serving["load_%.1f" % load] = serve_run(sreqs, load * value, Server)
+
+    small = None
+    if not args.no_percall:
+        sm = [r for r in stream_all if r[0].shape[0] <= 24][:256]
+        sb = B.DeviceBatch([B.prepare(A, loop=l) for A, l in sm])
+        sb.run(stream)
+        _, w = timed_runs(sb, 3, torch, stream)
+        rate, _ = per_call(sm, tw)
+        small = {"requests": len(sm), "batched": 3 * len(sm) / w, "per_call": rate,
+                 "batched_over_per_call": 3 * len(sm) / w / rate}
+        sb.close()
+
+    configs = None if args.no_configs else configs_run(torch, tw, B, stream)
     cpu = None if args.no_cpu else cpu_baseline(reqs)
+    head = serving["load_0.9"] if serving else {}
     line = {
-        "metric": "hafnians/s + p50/p99 ms, mixed n in [8,40] c128 stream vs per-call GPU",
+        "metric": METRIC,
         "value": value,
         "unit": "hafnians/s",
         "n_gpus": 1,
@@ -198,10 +268,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl, "requests_per_step": len(reqs), "parallelism": "single GPU"},
-        "per_call": per_call,
-        "batched_over_per_call": (value / per_call["value"]) if per_call else None,
-        "p50_ms": None,
-        "p99_ms": None,
+        "p50_ms": head.get("p50_ms"),
+        "p99_ms": head.get("p99_ms"),
+        "per_call": pc,
+        "batched_over_per_call": (value / pc["value"]) if pc else None,
+        "serving": serving,
+        "small_stream": small,
+        "configs": configs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

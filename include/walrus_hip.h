@@ -77,6 +77,38 @@ typedef struct wh_request {
  * (powers up to B^ceil(h/2), traces of higher powers as <B^a, (B^b)^T>). */
 #define WH_OPT_FULL 2
 
+/* ---- Batches over caller-owned device memory (no allocation, no host synchronisation).
+ * A batch is the host-side layout of n requests: kernel choice per request, term chunks and
+ * the job table.  The caller provides every device buffer, so batches can be enqueued on
+ * independent streams while others run (continuous batching). */
+typedef struct wh_batch wh_batch;
+
+/* Validates the requests and lays them out.  Host only: makes no HIP call. */
+int wh_batch_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_batch** batch);
+
+/* Sizes of the caller's buffers: job table bytes (device and pinned host), partial-sum slots
+ * (32 bytes each), and the batch's terms and algorithmic FP64 flops. */
+int wh_batch_info(const wh_batch* batch, size_t* job_table_bytes, int64_t* partial_slots,
+                  int64_t* terms, double* flops);
+
+/* Writes the job table (job_table_bytes) to host memory, e.g. a pinned staging buffer. */
+int wh_batch_write_jobs(const wh_batch* batch, void* host_dst);
+
+/* Enqueues on `stream`: the job-table copy h_jobs -> d_jobs when h_jobs is not NULL (h_jobs
+ * must be pinned and stay unchanged until the stream passes this point), the sieve kernels over
+ * `dpool`, and the fold writing 2n doubles to `dout` (request order).  d_jobs and d_partials
+ * must be 32-byte aligned and hold job_table_bytes / 32 * partial_slots bytes.  Calls that
+ * share any buffer race unless ordered on one stream. */
+int wh_batch_launch(const wh_batch* batch, const void* h_jobs, void* d_jobs, const double* dpool,
+                    double* d_partials, double* dout, void* stream);
+
+/* Per request (request order): first partial slot, number of slots, terms per slot. */
+int wh_batch_chunks(const wh_batch* batch, int64_t* first_chunk, int64_t* nchunks,
+                    int64_t* chunk_terms);
+
+int wh_batch_destroy(wh_batch* batch);
+
+/* ---- Plans: a batch plus device buffers owned by the library (convenience form). */
 typedef struct wh_plan wh_plan;
 
 /* Builds a launch plan for `n` requests on the current device: validates them, chooses the

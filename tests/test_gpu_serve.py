@@ -1,0 +1,84 @@
+"""GPU tests of the continuous-batching server: results are the same alone and in the mix."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from tests.conftest import random_symmetric
+
+import thewalrus_amd as tw
+from thewalrus_amd import workloads as W
+from thewalrus_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(seed=7, count=60):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        kind = k % 4
+        if kind == 0:
+            n = 2 * int(rng.integers(3, 13))
+            out.append(dict(A=random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop=False))
+        elif kind == 1:
+            n = int(rng.integers(5, 25))
+            out.append(dict(A=random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop=True))
+        elif kind == 2:
+            A, rpt, mu = W.config4(count=1, seed=int(rng.integers(1 << 30)))[0]
+            out.append(dict(A=A, rpt=rpt, mu=mu, loop=True))
+        else:
+            n = 2 * int(rng.integers(2, 4))  # n <= 6: includes host shortcut sizes
+            out.append(dict(A=random_symmetric(rng, n), loop=bool(rng.random() < 0.5)))
+    return out
+
+
+def _alone(r):
+    if "rpt" in r:
+        return tw.hafnian_repeated(r["A"], r["rpt"], mu=r["mu"], loop=r["loop"])
+    return tw.hafnian(r["A"], loop=r["loop"])
+
+
+def test_served_equals_alone_bitwise():
+    reqs = _requests()
+    with Server(slots=3, slot_pool_mb=2) as srv:
+        futs = [None] * len(reqs)
+
+        def submitter(idx):
+            rng = np.random.default_rng(idx)
+            for i in idx:
+                time.sleep(float(rng.uniform(0, 0.002)))
+                futs[i] = srv.submit(**reqs[i])
+
+        parts = [list(range(k, len(reqs), 3)) for k in range(3)]
+        threads = [threading.Thread(target=submitter, args=(p,)) for p in parts]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert srv.drain(timeout=120)
+        for r, f in zip(reqs, futs):
+            got = f.result(timeout=60)
+            want = _alone(r)
+            assert complex(got) == complex(want), (r["A"].shape[0], r.get("rpt"), got, want)
+        s = srv.stats.summary()
+        assert s["completed"] > 0 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_admission_by_footprint_queues_and_completes():
+    """A budget that admits about one n = 20 request at a time still completes everything."""
+    rng = np.random.default_rng(3)
+    mats = [random_symmetric(rng, 20, scale=0.2) for _ in range(6)]
+    with Server(slots=4, slot_pool_mb=1, hbm_budget_mb=0.02) as srv:
+        futs = [srv.submit(A) for A in mats]
+        assert srv.drain(timeout=120)
+        for A, f in zip(mats, futs):
+            assert complex(f.result()) == complex(tw.hafnian(A))
+        assert srv.stats.batches >= 3
+
+
+def test_oversized_request_rejected():
+    with Server(slots=1, slot_pool_mb=0.01) as srv:
+        with pytest.raises(ValueError):
+            srv.submit(np.ones((30, 30)))

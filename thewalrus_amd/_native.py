@@ -41,6 +41,12 @@ EXPORTS = [
     "wh_init",
     "wh_calc_hafnian",
     "wh_calc_loop_hafnian",
+    "wh_batch_create",
+    "wh_batch_info",
+    "wh_batch_write_jobs",
+    "wh_batch_launch",
+    "wh_batch_chunks",
+    "wh_batch_destroy",
     "wh_plan_create",
     "wh_plan_run",
     "wh_plan_info",
@@ -77,6 +83,14 @@ def load(path=None):
         L.wh_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.wh_plan_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
         L.wh_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_void_p)]
+        L.wh_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _ip, _ip, _dp]
+        L.wh_batch_write_jobs.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_batch_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.wh_batch_destroy.argtypes = [ctypes.c_void_p]
         L.wh_plan_partials.argtypes = [ctypes.c_void_p, _dp]
         L.wh_plan_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         for name in EXPORTS:

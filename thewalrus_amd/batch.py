@@ -48,6 +48,55 @@ def _is_real(*arrays):
     return all(a is None or not np.iscomplexobj(a) or not np.any(np.imag(a)) for a in arrays)
 
 
+def job_elements(job):
+    """complex128 elements a sieve job occupies in a device pool."""
+    twoE = job.Ax.shape[0]
+    return twoE * twoE + (twoE if job.loop else 0) + (twoE if job.oddloop is not None else 0)
+
+
+def pack_jobs(jobs, pool=None, options=0):
+    """Lays sieve jobs out in one complex128 pool.
+
+    Returns (requests, pool): a ctypes ``wh_request`` array with offsets into ``pool``.  When
+    ``pool`` (a writable complex128 array, e.g. a view of pinned memory) is given, the data
+    is written into it; otherwise a new array is allocated.
+    """
+    total = sum(job_elements(j) for j in jobs)
+    if pool is None:
+        pool = np.empty(max(total, 1), dtype=np.complex128)
+    elif pool.shape[0] < total:
+        raise ValueError("pool holds %d elements, %d needed" % (pool.shape[0], total))
+    reqs = (_native.WhRequest * max(len(jobs), 1))()
+    off = 0
+    for k, p in enumerate(jobs):
+        r = reqs[k]
+        twoE = p.Ax.shape[0]
+        E = twoE // 2
+        if E > _native.MAX_EDGES:
+            raise ValueError("%d edges exceed the device limit %d" % (E, _native.MAX_EDGES))
+        r.E = E
+        r.glynn = int(bool(p.glynn))
+        r.options = options
+        for i in range(E):
+            r.edge_reps[i] = int(p.edge_reps[i])
+        r.a_off = off
+        pool[off:off + twoE * twoE] = np.asarray(p.Ax, dtype=np.complex128).ravel()
+        off += twoE * twoE
+        r.d_off = r.v_off = -1
+        if p.loop:
+            r.d_off = off
+            pool[off:off + twoE] = p.Dx
+            off += twoE
+            if p.oddloop is not None:
+                r.v_off = off
+                pool[off:off + twoE] = p.oddV
+                off += twoE
+                r.oddloop[0] = complex(p.oddloop).real
+                r.oddloop[1] = complex(p.oddloop).imag
+        r.real = int(p.oddloop is None and _is_real(p.Ax, p.Dx))
+    return reqs, pool[:max(total, 1)]
+
+
 class DeviceBatch:
     """A set of prepared requests resident on one HIP device, ready to run as one plan.
 
@@ -68,56 +117,28 @@ class DeviceBatch:
         self.preps = list(preps)
         self.n = len(self.preps)
         self.host_values = {}
-        reqs = []
-        chunks = []
-        off = 0
         self.dev_index = []  # position in the device output for each request (or -1)
+        jobs = []
         for i, p in enumerate(self.preps):
             if not p.on_device:
                 self.host_values[i] = p.value
                 self.dev_index.append(-1)
                 continue
-            r = _native.WhRequest()
-            p = p.job
-            twoE = p.Ax.shape[0]
-            E = twoE // 2
-            if E > _native.MAX_EDGES:
-                raise ValueError("request %d: %d edges exceed the device limit %d" % (i, E, _native.MAX_EDGES))
-            r.E = E
-            r.glynn = int(bool(p.glynn))
-            r.options = (1 if force_generic else 0) | (2 if full_products else 0)
-            for k in range(E):
-                r.edge_reps[k] = int(p.edge_reps[k])
-            r.a_off = off
-            chunks.append(np.ascontiguousarray(p.Ax, dtype=np.complex128).ravel())
-            off += twoE * twoE
-            r.d_off = r.v_off = -1
-            if p.loop:
-                r.d_off = off
-                chunks.append(np.ascontiguousarray(p.Dx, dtype=np.complex128))
-                off += twoE
-                if p.oddloop is not None:
-                    r.v_off = off
-                    chunks.append(np.ascontiguousarray(p.oddV, dtype=np.complex128))
-                    off += twoE
-                    r.oddloop[0] = complex(p.oddloop).real
-                    r.oddloop[1] = complex(p.oddloop).imag
-            r.real = int(p.oddloop is None and _is_real(p.Ax, p.Dx))
-            self.dev_index.append(len(reqs))
-            reqs.append(r)
-        self.nreq = len(reqs)
+            self.dev_index.append(len(jobs))
+            jobs.append(p.job)
+        self.nreq = len(jobs)
         self.plan = None
         if self.nreq == 0:
             return
-        pool = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.complex128)
+        options = (1 if force_generic else 0) | (2 if full_products else 0)
+        reqs, pool = pack_jobs(jobs, options=options)
         with torch.cuda.device(self.device):
             self.pool = torch.from_numpy(pool.view(np.float64).copy()).to(self.device)
             self.out = torch.zeros(2 * self.nreq, dtype=torch.float64, device=self.device)
             L = _native.load()
-            arr = (_native.WhRequest * self.nreq)(*reqs)
-            self._reqs = arr
+            self._reqs = reqs
             plan = ctypes.c_void_p()
-            _native.check(L.wh_plan_create(arr, self.nreq, int(chunk_terms), ctypes.byref(plan)))
+            _native.check(L.wh_plan_create(reqs, self.nreq, int(chunk_terms), ctypes.byref(plan)))
             self.plan = plan
         terms = ctypes.c_int64()
         blocks = ctypes.c_int64()

@@ -312,6 +312,10 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
 
 }  // namespace
 
+struct wh_batch {
+    Layout layout;
+};
+
 struct wh_plan {
     int device = -1;
     int64_t n = 0;
@@ -334,6 +338,61 @@ int wh_init(int device) {
     HIP_TRY(hipSetDevice(device));
     Context* c;
     return ensure_context(device, &c);
+}
+
+int wh_batch_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_batch** batch) {
+    if (!reqs || n <= 0 || !batch) return fail(WH_EINVAL, "empty request list");
+    if (n > (int64_t)1 << 30) return fail(WH_EINVAL, "too many requests");
+    wh_batch* b = new wh_batch();
+    int rc = build_layout(reqs, n, chunk_terms, b->layout);
+    if (rc) { delete b; return rc; }
+    *batch = b;
+    return WH_OK;
+}
+
+int wh_batch_info(const wh_batch* b, size_t* job_table_bytes, int64_t* partial_slots, int64_t* terms,
+                  double* flops) {
+    if (!b) return fail(WH_EINVAL, "null batch");
+    if (job_table_bytes) *job_table_bytes = b->layout.jobs.size() * sizeof(wh_job_dev);
+    if (partial_slots) *partial_slots = b->layout.blocks;
+    if (terms) *terms = b->layout.terms;
+    if (flops) *flops = b->layout.flops;
+    return WH_OK;
+}
+
+int wh_batch_write_jobs(const wh_batch* b, void* host_dst) {
+    if (!b || !host_dst) return fail(WH_EINVAL, "null batch or buffer");
+    memcpy(host_dst, b->layout.jobs.data(), b->layout.jobs.size() * sizeof(wh_job_dev));
+    return WH_OK;
+}
+
+int wh_batch_launch(const wh_batch* b, const void* h_jobs, void* d_jobs, const double* dpool,
+                    double* d_partials, double* dout, void* stream) {
+    if (!b || !d_jobs || !dpool || !d_partials || !dout) return fail(WH_EINVAL, "null batch or buffer");
+    if (((uintptr_t)d_jobs | (uintptr_t)d_partials) & 31)
+        return fail(WH_EINVAL, "job table and partials must be 32-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    if (h_jobs)
+        HIP_TRY(hipMemcpyAsync(d_jobs, h_jobs, b->layout.jobs.size() * sizeof(wh_job_dev),
+                               hipMemcpyHostToDevice, s));
+    HIP_TRY(run_layout(b->layout, (const wh_job_dev*)d_jobs, dpool, (double4*)d_partials, dout, s));
+    return WH_OK;
+}
+
+int wh_batch_chunks(const wh_batch* b, int64_t* first_chunk, int64_t* nchunks, int64_t* chunk_terms) {
+    if (!b) return fail(WH_EINVAL, "null batch");
+    for (const wh_job_dev& j : b->layout.jobs) {
+        int64_t i = j.out_index;
+        if (first_chunk) first_chunk[i] = j.chunk_base;
+        if (nchunks) nchunks[i] = j.nchunks;
+        if (chunk_terms) chunk_terms[i] = j.chunk_terms;
+    }
+    return WH_OK;
+}
+
+int wh_batch_destroy(wh_batch* b) {
+    delete b;
+    return WH_OK;
 }
 
 int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_plan** plan) {
