@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--no-percall", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -175,6 +177,12 @@ def main():
     from thewalrus_amd.serve import Server
 
     torch.cuda.init()
+    if args.calibrate_copy_mb:
+        src = torch.ones(args.calibrate_copy_mb << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     stream_all = W.config5(count=max(args.requests, args.serve_requests, 512))
     reqs = [r for r in stream_all[: args.requests] if r[0].shape[0] >= args.nmin]
     wl = "mixed stream: even n uniform in [8,40], 50% loop, complex128 CN(0,1/n) symmetrised, seed 141"

@@ -10,7 +10,8 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwalrus_hip.so")
+# WH_LIB may name another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("WH_LIB") or os.path.join(_HERE, "libwalrus_hip.so")
 
 MAX_EDGES = 32
 MAX_PHOTONS = 128

@@ -53,38 +53,83 @@ __device__ __forceinline__ double col_sign(uint64_t t, int c, int h) {
 }
 
 
-// ---- tile helpers: lane (ti, tj) owns rows I0..I0+T-1, columns J0..J0+T-1 of the power
+// ---- tile helpers: lane (ti, tj) owns rows I0..I0+T-1, columns J0..J0+T-1 of the power.
+// LDS matrices are (8T) x (8T) with row stride 8T + 1 elements: the eight row tiles a
+// product step reads then start on distinct LDS banks.
+#ifndef WH_PM1_PAD
+#define WH_PM1_PAD 1
+#endif
+#ifndef WH_PM1_KUNROLL
+#define WH_PM1_KUNROLL 2
+#endif
+#ifndef WH_PM1_PIPELINE
+#define WH_PM1_PIPELINE 1
+#endif
+template <int T> struct Lds {
+    static constexpr int S = 8 * T + WH_PM1_PAD;
+};
 
 template <int T, class E>
 __device__ __forceinline__ void tile_store(E* Pw, const E (&acc)[T][T], int I0, int J0) {
 #pragma unroll
     for (int r = 0; r < T; ++r)
 #pragma unroll
-        for (int c = 0; c < T; ++c) Pw[(I0 + r) * (8 * T) + J0 + c] = acc[r][c];
+        for (int c = 0; c < T; ++c) Pw[(I0 + r) * Lds<T>::S + J0 + c] = acc[r][c];
 }
 
-// acc = (Pw Ms) diag(sign_J): one step of the power sequence, k summed in order
+// acc = (Pw Ms) diag(sign_J): one step of the power sequence, k summed in order.  The
+// operands of step k + 1 are loaded while step k's multiply-adds run (register double
+// buffer), so LDS latency is hidden even with one wave per SIMD.
+template <int T, class E>
+__device__ __forceinline__ void tile_load_k(const E* Pw, const E* Ms, int I0, int J0, int k, E (&av)[T],
+                                            E (&bv)[T]) {
+    constexpr int S = Lds<T>::S;
+#pragma unroll
+    for (int r = 0; r < T; ++r) av[r] = Pw[(I0 + r) * S + k];
+#pragma unroll
+    for (int c = 0; c < T; ++c) bv[c] = Ms[k * S + J0 + c];
+}
+
+template <int T, class E>
+__device__ __forceinline__ void tile_mac_k(E (&nacc)[T][T], const E (&av)[T], const E (&bv)[T]) {
+#pragma unroll
+    for (int r = 0; r < T; ++r)
+#pragma unroll
+        for (int c = 0; c < T; ++c) emac(nacc[r][c], av[r], bv[c]);
+}
+
 template <int T, class E>
 __device__ __forceinline__ void tile_product(const E* Pw, const E* Ms, E (&acc)[T][T], int I0, int J0,
                                              int n, const double (&sgnJ)[T]) {
-    constexpr int S = 8 * T;
     E nacc[T][T];
 #pragma unroll
     for (int r = 0; r < T; ++r)
 #pragma unroll
         for (int c = 0; c < T; ++c) ezero(nacc[r][c]);
-#pragma unroll 2
+#if WH_PM1_PIPELINE
+    E a0[T], b0[T], a1[T], b1[T];
+    tile_load_k<T>(Pw, Ms, I0, J0, 0, a0, b0);
+    int k = 0;
+    for (; k + 2 <= n; k += 2) {
+        tile_load_k<T>(Pw, Ms, I0, J0, k + 1, a1, b1);
+        tile_mac_k<T>(nacc, a0, b0);
+        // unconditional prefetch (the last one re-reads row k, unused) keeps the operand
+        // registers free of branch merges
+        const int kn = k + 2 < n ? k + 2 : k;
+        tile_load_k<T>(Pw, Ms, I0, J0, kn, a0, b0);
+        tile_mac_k<T>(nacc, a1, b1);
+    }
+    if (k < n) tile_mac_k<T>(nacc, a0, b0);
+    // keep later LDS reads (transposed tiles) from being hoisted into the loop's live range
+    asm volatile("" ::: "memory");
+#else
+#pragma unroll WH_PM1_KUNROLL
     for (int k = 0; k < n; ++k) {
         E av[T], bv[T];
-#pragma unroll
-        for (int r = 0; r < T; ++r) av[r] = Pw[(I0 + r) * S + k];
-#pragma unroll
-        for (int c = 0; c < T; ++c) bv[c] = Ms[k * S + J0 + c];
-#pragma unroll
-        for (int r = 0; r < T; ++r)
-#pragma unroll
-            for (int c = 0; c < T; ++c) emac(nacc[r][c], av[r], bv[c]);
+        tile_load_k<T>(Pw, Ms, I0, J0, k, av, bv);
+        tile_mac_k<T>(nacc, av, bv);
     }
+#endif
 #pragma unroll
     for (int r = 0; r < T; ++r)
 #pragma unroll
@@ -110,7 +155,7 @@ __device__ __forceinline__ E tile_tdot(const E* X, const E (&acc)[T][T], int I0,
 #pragma unroll
     for (int c = 0; c < T; ++c)
 #pragma unroll
-        for (int r = 0; r < T; ++r) emac(s, acc[r][c], X[(J0 + c) * (8 * T) + I0 + r]);
+        for (int r = 0; r < T; ++r) emac(s, acc[r][c], X[(J0 + c) * Lds<T>::S + I0 + r]);
     return s;
 }
 
@@ -169,7 +214,7 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
                                          unsigned char* smem) {
     using E = typename Elem<REAL>::T;
     constexpr int NP = 8 * T;  // padded side
-    constexpr int S = NP;      // LDS row stride (elements)
+    constexpr int S = Lds<T>::S;  // LDS row stride (elements)
     const int nw = blockDim.x / WH_WAVE;
     const int wave = threadIdx.x / WH_WAVE;
     const int lane = threadIdx.x % WH_WAVE;
@@ -242,14 +287,16 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
 #pragma unroll
             for (int c = 0; c < T; ++c) acc[r][c] = escale(Ms[(I0 + r) * S + J0 + c], sgnJ[c]);
 
-        // loop: per-lane row weights XD_d[I] and column weights D[J]
-        E xdI[T], dJ[T];
-        if (LOOP) {
+        // loop: per-lane row weights XD_d[I] and column weights D[J], read from LDS where
+        // used so they do not occupy registers across the products
+        auto load_xd = [&](E (&xdI)[T]) {
 #pragma unroll
             for (int r = 0; r < T; ++r) xdI[r] = escale(XD0[I0 + r], col_sign(tu, I0 + r, h));
+        };
+        auto load_d = [&](E (&dJ)[T]) {
 #pragma unroll
             for (int c = 0; c < T; ++c) dJ[c] = Dv[J0 + c];
-        }
+        };
 
         if constexpr (!HALF) {
             // the reference's order: B^(a+1) = B^a B for a < h, p_a = tr(B^a)
@@ -257,7 +304,12 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
                 E tr = tile_trace<T>(acc, ti == tj);
                 E bl;
                 ezero(bl);
-                if (LOOP && a <= h - 1) bl = tile_bilinear<T>(acc, xdI, dJ);
+                if (LOOP && a <= h - 1) {
+                    E xdI[T], dJ[T];
+                    load_xd(xdI);
+                    load_d(dJ);
+                    bl = tile_bilinear<T>(acc, xdI, dJ);
+                }
                 cplx trc = wave_csum(etoc(tr));
                 cplx blc = LOOP ? wave_csum(etoc(bl)) : cmk(0.0, 0.0);
                 if (lane == 0) {
@@ -300,7 +352,9 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
                     pt[ke] = qec;  // ke <= h + 1 < WH_EMAX + 2
                 }
                 if (LOOP) {
-                    E y[T], z[T];
+                    E y[T], z[T], xdI[T], dJ[T];
+                    load_xd(xdI);
+                    load_d(dJ);
                     tile_yvec<T>(acc, xdI, y);
                     tile_zvec<T>(acc, dJ, z);
                     E de, dodd;
@@ -389,9 +443,9 @@ wh_sieve_pm1(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restr
 
 size_t wh_pm1_lds_bytes(int T, bool real, int nw) {
     size_t esz = real ? sizeof(double) : sizeof(cplx);
-    size_t NP = 8 * (size_t)T;
-    return (1 + nw) * NP * NP * esz + 2 * NP * esz + nw * 4 * (WH_EMAX + 2) * sizeof(cplx) +
-           nw * sizeof(double4);
+    size_t NP = 8 * (size_t)T, S = NP + WH_PM1_PAD;
+    size_t mat = (NP * S * esz + 31) / 32 * 32;  // keeps the later regions 32-byte aligned
+    return (1 + nw) * mat + 2 * NP * esz + nw * 4 * (WH_EMAX + 2) * sizeof(cplx) + nw * sizeof(double4) + 32;
 }
 
 template <int T, bool REAL, bool LOOP, bool HALF>
