@@ -1,0 +1,166 @@
+"""Gaussian-state probability requests (thewalrus_amd.quantum) against the reference's own golden
+values (thewalrus/tests/test_quantum.py) and closed forms.  Conversions run on the CPU; every
+test that evaluates a sieve is marked gpu."""
+import math
+
+import numpy as np
+import pytest
+
+from thewalrus_amd import quantum as q
+
+# test_quantum.py:285-297
+V = np.array(
+    [
+        [0.6964938, 0.06016962, -0.01970064, 0.03794393, 0.07913992, -0.08890985],
+        [0.06016962, 0.85435861, -0.01648842, 0.10493462, 0.01223525, 0.12484726],
+        [-0.01970064, -0.01648842, 0.89450003, -0.13182502, 0.13529134, -0.10621978],
+        [0.03794393, 0.10493462, -0.13182502, 1.47820656, -0.11611807, 0.05634905],
+        [0.07913992, 0.01223525, 0.13529134, -0.11611807, 1.20819301, -0.0061647],
+        [-0.08890985, 0.12484726, -0.10621978, 0.05634905, -0.0061647, 1.1636695],
+    ]
+)
+MU = np.array([0.04948628, -0.55738964, 0.71298259, 0.17728629, -0.14381673, 0.33340778])
+
+# test_quantum.py:278-282 (displaced) and :313-317 (no displacement)
+DISP = [
+    ([[0, 0, 0], [0, 0, 0]], 0.7304280085350833),
+    ([[1, 0, 1], [0, 0, 0]], -0.009290003060522444 + 0.002061369459502776j),
+    ([[1, 0, 1], [0, 0, 1]], -0.004088994220552936 - 0.0009589367814578206j),
+    ([[0, 2, 0], [0, 0, 0]], 0.003384487265468196 - 0.03127114305387707j),
+    ([[0, 2, 0], [0, 2, 3]], -8.581668587044574e-05 - 6.134980446713632e-05j),
+]
+NODISP = [
+    ([[0, 0, 0], [0, 0, 0]], 0.9645169885669383),
+    ([[1, 0, 1], [0, 0, 0]], -0.016156769991363732 + 0.05039373212461916j),
+    ([[1, 0, 1], [0, 0, 1]], 0),
+    ([[0, 2, 0], [0, 0, 0]], -0.05911275266690908 - 0.0049431163436861j),
+    ([[0, 2, 0], [0, 2, 3]], 0),
+]
+
+
+def two_mode_squeezing(r, phi):
+    """Symplectic two-mode squeezer (the reference's symplectic.two_mode_squeezing layout)."""
+    cp, sp, ch, sh = np.cos(phi), np.sin(phi), np.cosh(r), np.sinh(r)
+    return np.array(
+        [
+            [ch, cp * sh, 0, sp * sh],
+            [cp * sh, ch, sp * sh, 0],
+            [0, sp * sh, ch, -cp * sh],
+            [sp * sh, 0, -cp * sh, ch],
+        ]
+    )
+
+
+def test_conversions_roundtrip():
+    Q = q.Qmat(V)
+    assert np.allclose(q.Covmat(Q), V)
+    beta = q.complex_to_real_displacements(MU)
+    assert np.allclose(q.real_to_complex_displacements(beta), MU)
+    A = q.Amat(V)
+    assert np.allclose(A, A.T)  # symmetric, as a hafnian argument must be
+    assert q.is_valid_cov(V) and q.is_pure_cov(V)  # the reference's V has unit purity
+    assert q.is_pure_cov(np.identity(4))
+    thermal = np.diag([2.0, 3.0, 2.0, 3.0])
+    assert q.is_valid_cov(thermal) and not q.is_pure_cov(thermal)
+    assert not q.is_valid_cov(np.diag([0.5, 0.5]))  # violates the uncertainty relation
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("t", DISP)
+def test_density_matrix_element_disp(t):
+    beta = q.complex_to_real_displacements(MU)
+    Q = q.Qmat(V)
+    el, ex = t
+    res = q.density_matrix_element(q.real_to_complex_displacements(beta), q.Covmat(Q), el[0], el[1])
+    assert np.allclose(ex, res)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("t", NODISP)
+def test_density_matrix_element_no_disp(t):
+    beta = q.complex_to_real_displacements(np.zeros([6]))
+    Q = q.Qmat(V)
+    el, ex = t
+    res = q.density_matrix_element(q.real_to_complex_displacements(beta), q.Covmat(Q), el[0], el[1])
+    assert np.allclose(ex, res)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("i", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("j", [0, 1, 2, 3, 4])
+def test_pure_state_amplitude_two_mode_squeezed(i, j):
+    """test_quantum.py:582-597."""
+    nbar, phase = 1.0, np.pi / 8
+    r = np.arcsinh(np.sqrt(nbar))
+    cov = two_mode_squeezing(2 * r, phase)
+    exact = 0.0 if i != j else np.exp(1j * i * phase) * (nbar / (1.0 + nbar)) ** (i / 2) / np.sqrt(1.0 + nbar)
+    assert np.allclose(exact, q.pure_state_amplitude(np.zeros([4], dtype=complex), cov, [i, j]))
+
+
+@pytest.mark.parametrize("i", [0, 1, 2, 3, 4])
+def test_pure_state_amplitude_coherent(i):
+    """test_quantum.py:600-609 (one mode: resolved on the host)."""
+    cov = np.identity(2)
+    mu = np.array([1.0, 2.0])
+    alpha = q.complex_to_real_displacements(mu)[0]
+    exact = np.exp(-0.5 * np.abs(alpha) ** 2) * alpha**i / np.sqrt(math.factorial(i))
+    assert np.allclose(exact, q.pure_state_amplitude(mu, cov, [i]))
+
+
+@pytest.mark.gpu
+def test_pure_state_amplitude_squeezed_coherent():
+    """test_quantum.py:612-630."""
+    r = 0.43
+    mu = np.array([0.24, -0.2])
+    V1 = np.diag(np.array(np.exp([-2 * r, 2 * r])))
+    amps = np.array([q.pure_state_amplitude(mu, V1, [i]) for i in range(5)])
+    expected = np.array(
+        [[0.89054874, 0.15018085 + 0.05295904j, -0.23955467 + 0.01263025j, -0.0734589 - 0.02452154j, 0.07862323 - 0.00868528j],
+         [0.15018085 - 0.05295904j, 0.02847564, -0.03964706 + 0.01637575j, -0.01384625 + 0.00023317j, 0.01274241 - 0.00614023j],
+         [-0.23955467 - 0.01263025j, -0.03964706 - 0.01637575j, 0.06461854, 0.01941242 + 0.00763805j, -0.02127257 + 0.00122123j],
+         [-0.0734589 + 0.02452154j, -0.01384625 - 0.00023317j, 0.01941242 - 0.00763805j, 0.00673463, -0.00624626 + 0.00288134j],
+         [0.07862323 + 0.00868528j, 0.01274241 + 0.00614023j, -0.02127257 - 0.00122123j, -0.00624626 - 0.00288134j, 0.00702606]]
+    )
+    assert np.allclose(expected, np.outer(amps, amps.conj()))
+
+
+@pytest.mark.gpu
+def test_pure_amplitude_tms_complex_displacement():
+    """test_quantum.py:633-657: two-mode squeezed displaced state vs the Laguerre closed form."""
+    r, qv, p, hbar, n = 0.8, 0.6, -1.4, 2, 3
+    alpha = 1 / np.sqrt(2 * hbar) * (qv + 1j * p)
+    alpha_c = np.conjugate(alpha)
+    S = two_mode_squeezing(-r, 0)
+    cov = hbar / 2 * S @ S.T
+    mu = np.array([qv, qv, p, p])
+    amp1 = q.pure_state_amplitude(mu, cov, [n, n], hbar=hbar)
+    hyperbolic_pref = (-np.tanh(r)) ** n / np.cosh(r)
+    laguerre_part = np.polynomial.Laguerre([0] * n + [1])((alpha + alpha_c * np.tanh(r)) ** 2 / np.tanh(r))
+    exp_part = np.exp(-(alpha * alpha_c + alpha_c**2 * np.tanh(r)))
+    assert np.allclose(amp1, hyperbolic_pref * laguerre_part * exp_part)
+
+
+@pytest.mark.gpu
+def test_probabilities_mixed_batched_equals_elements():
+    """All density-matrix diagonals in one device batch = element by element; total <= 1."""
+    mu = MU
+    cov = V + 0.3 * np.identity(6)  # a mixed state (purity < 1)
+    assert not q.is_pure_cov(cov)
+    probs = q.probabilities(mu, cov, cutoff=4)
+    assert probs.shape == (4, 4, 4)
+    for i in [(0, 0, 0), (1, 0, 2), (3, 3, 3), (2, 1, 0)]:
+        el = q.density_matrix_element(mu, cov, list(i), list(i))
+        assert np.isclose(probs[i], max(0.0, np.real_if_close(el)), rtol=1e-10, atol=1e-14)
+    assert 0.5 < probs.sum() <= 1.0 + 1e-9
+
+
+@pytest.mark.gpu
+def test_probabilities_pure_two_mode_squeezed():
+    """Thermal photon statistics of one arm: P(n, n) = nbar^n / (1 + nbar)^(n+1)."""
+    nbar = 0.5
+    r = np.arcsinh(np.sqrt(nbar))
+    cov = two_mode_squeezing(2 * r, 0.3)
+    probs = q.probabilities(np.zeros(4), cov, cutoff=5)
+    for n in range(5):
+        assert np.isclose(probs[n, n], nbar**n / (1 + nbar) ** (n + 1))
+    assert np.allclose(probs - np.diag(np.diag(probs)), 0)
