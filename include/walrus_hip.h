@@ -55,6 +55,28 @@ int wh_calc_hafnian(const double* A, int64_t twoE, const int64_t* edge_reps, int
 int wh_calc_loop_hafnian(const double* A, const double* D, const int64_t* edge_reps, int64_t E,
                          const double* oddloop, const double* oddV, int glynn, double* out);
 
+/* Replaces _calc_loop_hafnian_batch_even(A, D, fixed_edge_reps, batch_max, odd_cutoff, glynn)
+ * (thewalrus/loop_hafnian_batch.py:52-123) when odd_variant = 0 and
+ * _calc_loop_hafnian_batch_odd(A, D, fixed_edge_reps, batch_max, even_cutoff, glynn) (:128-208)
+ * when odd_variant = 1.  A: 2E x 2E complex128 and D: 2E complex128, ordered as
+ * loop_hafnian_batch (:260-304) orders them (edge 0 the batched mode, edge 1 the odd mode for
+ * odd_variant = 1, then the Ef fixed edges), with E = Ef + 1 + odd_variant.  cutoff: the
+ * odd_cutoff / even_cutoff argument (0 or 1).  out: out_len complex values (2 doubles each).
+ * Returns the number of outputs written, L = 2 batch_max + cutoff + 1 + odd_variant (the loop
+ * hafnians for N_det = 0..L-1 photons in the batched mode), or a negative error code.
+ * Limits: 2E <= 48, L <= 128, multiplicities <= 128, photons 2 sum(fixed) + odd_variant + L - 1 <= 128.  Synchronous. */
+int wh_loop_hafnian_batch(const double* A, const double* D, int64_t twoE, const int64_t* fixed_edge_reps,
+                          int64_t Ef, int64_t batch_max, int64_t cutoff, int odd_variant, int glynn,
+                          double* out, int64_t out_len);
+
+/* Replaces _calc_loop_hafnian_batch_gamma_even / _odd (thewalrus/loop_hafnian_batch_gamma.py
+ * :53-129, :132-219): the same sieve for nD diagonals sharing A (D: nD x 2E complex128,
+ * row-major), one per fanout row of the threshold sampler.  out: nD x L complex values, row kd
+ * equal to wh_loop_hafnian_batch with D row kd.  Returns nD L (<= 128) or a negative error. */
+int wh_loop_hafnian_batch_gamma(const double* A, const double* D, int64_t nD, int64_t twoE,
+                                const int64_t* fixed_edge_reps, int64_t Ef, int64_t batch_max, int64_t cutoff,
+                                int odd_variant, int glynn, double* out, int64_t out_len);
+
 /* One prepared request for the batched path.  Offsets count complex128 elements from the
  * start of a device-resident pool that holds every request's reordered matrix (and vectors). */
 typedef struct wh_request {

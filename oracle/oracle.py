@@ -42,6 +42,8 @@ def lib():
         L.wo_fcoeffs.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         L.wo_brute_force.argtypes = [dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
         L.wo_kan.argtypes = [dp, dp, ip, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
+        L.wo_loop_hafnian_batch.argtypes = [dp, dp, ctypes.c_int64, ip, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         _LIB = L
     return _LIB
 
@@ -252,3 +254,63 @@ def double_factorial(n):
 
 
 TELEPHONE = [1, 1, 2, 4, 10, 26, 76, 232, 764, 2620, 9496]  # test_hafnian.py:30
+
+
+# --------------------------------------------------------------------------------------
+# loop_hafnian_batch (loop_hafnian_batch.py:211-304), host preparation restated
+# --------------------------------------------------------------------------------------
+def _batch_edges_even(fixed_edges):
+    """loop_hafnian_batch.py:211-231: the batched mode paired with itself as edge 0."""
+    if len(fixed_edges) == 0:
+        return np.array([0, 0], dtype=int)
+    n_edges = len(fixed_edges)
+    new = max(fixed_edges) + 1
+    half = n_edges // 2
+    return np.array([new] + list(fixed_edges[:half]) + [new] + list(fixed_edges[half:]), dtype=int)
+
+
+def _batch_edges_odd(fixed_edges, oddmode):
+    """loop_hafnian_batch.py:234-257: edge 0 the batched mode, edge 1 (odd mode, batched mode)."""
+    if len(fixed_edges) == 0:
+        return np.array([1, oddmode, 1, 1], dtype=int)
+    n_edges = len(fixed_edges)
+    new = max(max(fixed_edges), oddmode) + 1
+    half = n_edges // 2
+    return np.array([new, oddmode] + list(fixed_edges[:half]) + [new, new] + list(fixed_edges[half:]),
+                    dtype=int)
+
+
+def loop_hafnian_batch(A, D, fixed_reps, N_cutoff, glynn=True, precision="double"):
+    """Loop hafnians for every photon number 0..N_cutoff of the last mode
+    (loop_hafnian_batch.py:260-304)."""
+    A = np.asarray(A)
+    D = np.asarray(D)
+    nz = np.nonzero(list(fixed_reps) + [1])[0]
+    Anz = A[np.ix_(nz, nz)]
+    Dnz = D[nz]
+    fixed_reps_nz = np.asarray(fixed_reps)[nz[:-1]]
+    fixed_edges, fixed_m, odd = pair_modes(fixed_reps_nz)
+    if odd is None:
+        batch_max, cutoff, variant = N_cutoff // 2, N_cutoff % 2, 0
+        edges = _batch_edges_even(fixed_edges)
+    else:
+        batch_max, cutoff, variant = (N_cutoff - 1) // 2, 1 - (N_cutoff % 2), 1
+        edges = _batch_edges_odd(fixed_edges, odd)
+    Ax = _c(Anz[np.ix_(edges, edges)])
+    Dx = _c(Dnz[edges])
+    fm = np.ascontiguousarray(fixed_m, dtype=np.int64)
+    p = _prec(precision)
+    out = np.zeros(4 * (N_cutoff + 4), dtype=np.float64)
+    L = lib().wo_loop_hafnian_batch(_dp(Ax), _dp(Dx), Ax.shape[0], _ip(fm) if fm.size else None, fm.size, batch_max,
+                                    cutoff, variant, int(bool(glynn)), p, _dp(out))
+    if p == 0:
+        return out[: 2 * L].view(np.complex128).copy()
+    o = out[: 4 * L].reshape(-1, 4)
+    return (o[:, 0] + o[:, 1]) + 1j * (o[:, 2] + o[:, 3])
+
+
+def loop_hafnian_batch_gamma(A, D, fixed_reps, N_cutoff, glynn=True, precision="double"):
+    """loop_hafnian_batch_gamma.py:222-269: row k is the batch with diagonal D[k] (the reference
+    shares each term's matrix between rows; every row's arithmetic is that of the batch)."""
+    D = np.atleast_2d(np.asarray(D))
+    return np.array([loop_hafnian_batch(A, D[k], fixed_reps, N_cutoff, glynn, precision) for k in range(D.shape[0])])

@@ -32,6 +32,7 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <numeric>
 #include <algorithm>
 #include <omp.h>
 
@@ -360,15 +361,19 @@ static cvec<T> f_loop_odd(cvec<T>& AX, const cvec<T>& AX_S, const cvec<T>& XD, c
 }
 
 // _hafnian.py:27-45 nb_binom (exact in int64 for the sizes used here)
+// _hafnian.py:27-45 nb_binom.  The reference multiplies before dividing in int64, which wraps
+// for n >= 62; dividing out the common factor first keeps every intermediate <= the result.
 static int64_t binom_i(int64_t n, int64_t k) {
     if (k < 0 || k > n) return 0;
     if (k == 0 || k == n) return 1;
-    int64_t b = 1;
+    uint64_t b = 1;
     for (int64_t i = 0; i < std::min(k, n - k); ++i) {
-        b *= n - i;
-        b /= i + 1;
+        uint64_t num = (uint64_t)(n - i), den = (uint64_t)(i + 1);
+        uint64_t g = std::gcd(b, den);
+        b /= g;
+        b *= num / (den / g);
     }
-    return b;
+    return (int64_t)b;
 }
 
 // _hafnian.py:163-180 find_kept_edges: digit 0 is the most significant, base e_i + 1
@@ -745,3 +750,95 @@ int wo_kan(const double* B, const double* u, const int64_t* r, int64_t n, int lo
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// loop_hafnian_batch.py restatement (GBS sampler core)
+// ---------------------------------------------------------------------------------------
+
+// _calc_loop_hafnian_batch_even (loop_hafnian_batch.py:52-123) when odd_variant == 0 and
+// _calc_loop_hafnian_batch_odd (:128-208) when odd_variant == 1.  `cutoff` is odd_cutoff
+// (even variant) or even_cutoff (odd variant).  The reference hands the same matrix copy to
+// successive powertrace calls, so later calls see the Hessenberg form left by earlier ones;
+// that order is kept.  Output length: 2 batch_max + cutoff + 1 (+1 for the odd variant).
+template <class T>
+static std::vector<cx<T>> lhb(const cvec<T>& A, const cvec<T>& D, int twoE, const int64_t* fixed,
+                              int Ef, int batch_max, int cutoff, bool odd_variant, bool glynn) {
+    int E = Ef + (odd_variant ? 2 : 1);
+    std::vector<int64_t> e(E);
+    e[0] = batch_max;
+    if (odd_variant) e[1] = 1;
+    for (int i = 0; i < Ef; ++i) e[(odd_variant ? 2 : 1) + i] = fixed[i];
+    int64_t fsum = 0;
+    for (int i = 0; i < Ef; ++i) fsum += fixed[i];
+    int N_fixed = (int)(2 * fsum) + (odd_variant ? 1 : 0);
+    int N_max = odd_variant ? N_fixed + 2 * batch_max + cutoff + 1 : N_fixed + 2 * batch_max + cutoff;
+    int L = 2 * batch_max + cutoff + (odd_variant ? 2 : 1);
+    int64_t steps = 1;
+    for (int i = 0; i < E; ++i) steps *= e[i] + 1;
+    int64_t emax = 0;
+    for (int i = 0; i < E; ++i) emax = std::max(emax, e[i]);
+    int mb = (int)emax + cutoff;  // precompute_binoms(max(edge_reps) + cutoff)
+    std::vector<int64_t> binoms((size_t)(mb + 1) * (mb + 1));
+    for (int i = 0; i <= mb; ++i)
+        for (int j = 0; j <= mb; ++j) binoms[(size_t)i * (mb + 1) + j] = binom_i(i, j);
+    auto BN = [&](int64_t a, int64_t b) { return (double)binoms[(size_t)a * (mb + 1) + b]; };
+
+    cvec<T> oddV(twoE), oddV0(twoE);
+    for (int c = 0; c < twoE; ++c) oddV[c] = A[c];  // A[0, :]
+    if (odd_variant)
+        for (int c = 0; c < twoE; ++c) oddV0[c] = A[(size_t)twoE + c];  // A[1, :]
+    cx<T> oddloop = D[0], oddloop0 = odd_variant ? D[1] : cx<T>();
+
+    std::vector<cx<T>> H(L);
+    std::vector<int64_t> k(E), delta(E);
+    TermMats<T> tm, tm0;
+    for (int64_t j = 0; j < steps; ++j) {
+        kept_edges(j, e.data(), E, k.data());
+        int64_t edges_sum = 0;
+        for (int i = 0; i < E; ++i) edges_sum += k[i];
+        double binom_prod = 1.0;
+        for (int i = 1; i < E; ++i) binom_prod *= BN(e[i], k[i]);
+        for (int i = 0; i < E; ++i) delta[i] = glynn ? 2 * k[i] - e[i] : k[i];
+        term_matrices(A, twoE, delta.data(), E, &D, &oddV, tm);
+        cvec<T> AXc(tm.AX);
+        std::vector<cx<T>> Hnew(L);
+        if (odd_variant && k[0] == 0 && k[1] == 0) {
+            // get_submatrix_batch_odd0 (_hafnian.py:360-380)
+            term_matrices(A, twoE, delta.data(), E, &D, &oddV0, tm0);
+            double pm = (((N_fixed / 2 - edges_sum) % 2) != 0) ? -1.0 : 1.0;
+            cx<T> f = f_loop_odd(AXc, tm.AX, tm.XD, tm.D, tm.m, N_fixed, oddloop0, tm0.oddVX)[N_fixed];
+            H[0] += mulr(f, T(binom_prod * pm));
+        }
+        cvec<T> f_even = f_loop(AXc, tm.AX, tm.XD, tm.D, tm.m, N_max);
+        cvec<T> f_odd = f_loop_odd(AXc, tm.AX, tm.XD, tm.D, tm.m, N_max, oddloop, tm.oddVX);
+        int start = (int)(2 * k[0]) + (odd_variant ? 1 : 0);
+        for (int N_det = start; N_det < L; ++N_det) {
+            int N = N_fixed + N_det;
+            double pm = (((N / 2 - edges_sum) % 2) != 0) ? -1.0 : 1.0;
+            int half = odd_variant ? (N_det - 1) / 2 : N_det / 2;
+            double nb = BN(half, k[0]) * binom_prod;
+            bool use_even = odd_variant ? (N % 2 == 0) : (N_det % 2 == 0);
+            cx<T> f = use_even ? f_even[N / 2] : f_odd[N];
+            Hnew[N_det] += mulr(f, T(nb * pm));
+        }
+        for (int q = 0; q < L; ++q) H[q] += Hnew[q];
+    }
+    if (glynn)
+        for (int q = 0; q < L; ++q) H[q] = mulr(H[q], T(std::ldexp(1.0, -((N_fixed + q) / 2))));
+    return H;
+}
+
+extern "C" int wo_loop_hafnian_batch(const double* A, const double* D, int64_t twoE, const int64_t* fixed,
+                                     int64_t Ef, int64_t batch_max, int64_t cutoff, int odd_variant, int glynn,
+                                     int precision, double* out) {
+    if (precision == 0) {
+        auto H = lhb<double>(load<double>(A, (size_t)twoE * twoE), load<double>(D, twoE), (int)twoE, fixed,
+                             (int)Ef, (int)batch_max, (int)cutoff, odd_variant != 0, glynn != 0);
+        for (size_t q = 0; q < H.size(); ++q) store(H[q], out + 2 * q);
+        return (int)H.size();
+    }
+    auto H = lhb<dd>(load<dd>(A, (size_t)twoE * twoE), load<dd>(D, twoE), (int)twoE, fixed, (int)Ef,
+                     (int)batch_max, (int)cutoff, odd_variant != 0, glynn != 0);
+    for (size_t q = 0; q < H.size(); ++q) store(H[q], out + 4 * q);
+    return (int)H.size();
+}

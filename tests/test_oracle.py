@@ -288,3 +288,49 @@ def test_thread_count_invariance():
     v1 = O.hafnian_sieve(A, nthreads=1)
     v8 = O.hafnian_sieve(A, nthreads=8)
     assert relerr(v1, v8) < 1e-12
+
+
+# ---- loop_hafnian_batch (GBS sampler core): every output is one loop hafnian, pinned by Kan
+LHB_CASES = [
+    (4, [1, 2, 0], 5),   # paired fixed modes (even variant), a zero repetition dropped
+    (4, [1, 1, 1], 4),   # one mode left unpaired (odd variant)
+    (3, [0, 0], 3),      # no fixed photons: edges [0, 0]
+    (2, [1], 4),         # odd variant with no fixed edge: edges [1, odd, 1, 1]
+    (5, [2, 0, 1, 1], 6),
+    (6, [3, 1, 0, 2, 1], 7),
+]
+
+
+@pytest.mark.parametrize("n,fixed,N_cutoff", LHB_CASES)
+@pytest.mark.parametrize("glynn", [True, False])
+@pytest.mark.parametrize("precision", ["double", "dd"])
+def test_loop_hafnian_batch_vs_kan(n, fixed, N_cutoff, glynn, precision):
+    """Output q is lhaf of A with repetitions fixed + [q] and diagonal D (Kan's formula)."""
+    rng = np.random.default_rng(1000 + 7 * n + N_cutoff)
+    A = random_symmetric(rng, n, scale=0.6)
+    D = rng.normal(size=n) + 1j * rng.normal(size=n)
+    H = O.loop_hafnian_batch(A, D, fixed, N_cutoff, glynn=glynn, precision=precision)
+    assert H.shape == (N_cutoff + 1,)
+    for q in range(N_cutoff + 1):
+        k = O.kan(A, list(fixed) + [q], mu=D, loop=True)
+        assert relerr(H[q], k) < (1e-11 if precision == "double" else 1e-14), q
+
+
+def test_loop_hafnian_batch_odd_zero_cutoff():
+    """Odd variant at N_cutoff = 0: batch_max = -1 leaves no term, the result is [0]
+    (loop_hafnian_batch.py:298-304 with steps = prod(edge_reps + 1) = 0)."""
+    rng = np.random.default_rng(5)
+    A = random_symmetric(rng, 3)
+    D = rng.normal(size=3) + 0j
+    H = O.loop_hafnian_batch(A, D, [1, 0], 0)
+    assert H.shape == (1,) and H[0] == 0
+
+
+def test_loop_hafnian_batch_gamma_rows_vs_kan():
+    rng = np.random.default_rng(77)
+    A = random_symmetric(rng, 4, scale=0.5)
+    D = rng.normal(size=(3, 4)) + 1j * rng.normal(size=(3, 4))
+    H = O.loop_hafnian_batch_gamma(A, D, [1, 0, 2], 3, precision="dd")
+    for k in range(3):
+        for q in range(4):
+            assert relerr(H[k, q], O.kan(A, [1, 0, 2, q], mu=D[k], loop=True)) < 1e-14

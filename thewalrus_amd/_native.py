@@ -54,6 +54,8 @@ EXPORTS = [
     "wh_plan_partials",
     "wh_plan_chunks",
     "wh_plan_destroy",
+    "wh_loop_hafnian_batch",
+    "wh_loop_hafnian_batch_gamma",
 ]
 
 _lib = None
@@ -94,6 +96,11 @@ def load(path=None):
         L.wh_batch_destroy.argtypes = [ctypes.c_void_p]
         L.wh_plan_partials.argtypes = [ctypes.c_void_p, _dp]
         L.wh_plan_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.wh_loop_hafnian_batch.argtypes = [_dp, _dp, ctypes.c_int64, _ip, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64]
+        L.wh_loop_hafnian_batch_gamma.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _ip, ctypes.c_int64,
+                                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp,
+                                                  ctypes.c_int64]
         for name in EXPORTS:
             getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
         if path is None:
@@ -137,3 +144,33 @@ def calc_loop_hafnian(Ax, Dx, edge_reps, oddloop=None, oddV=None, glynn=True):
     check(L.wh_calc_loop_hafnian(_dptr(Ax), _dptr(Dx), e.ctypes.data_as(_ip), e.shape[0], _dptr(ol),
                                  _dptr(oV), int(bool(glynn)), _dptr(out)))
     return np.complex128(complex(out[0], out[1]))
+
+
+def loop_hafnian_batch(Ax, Dx, fixed_edge_reps, batch_max, cutoff, odd_variant, glynn=True):
+    """Device sieve at the reference seams ``_calc_loop_hafnian_batch_even`` /
+    ``_calc_loop_hafnian_batch_odd`` (``loop_hafnian_batch.py:52``, ``:128``); with a 2-D ``Dx``
+    (one diagonal per row) ``_calc_loop_hafnian_batch_gamma_even`` / ``_odd``
+    (``loop_hafnian_batch_gamma.py:53``, ``:132``), returning one row per diagonal.
+
+    ``batch_max < 0`` (the odd variant at ``N_cutoff = 0``) gives the reference's empty term
+    range: zeros."""
+    L = load()
+    Ax = np.ascontiguousarray(Ax, dtype=np.complex128)
+    Dx = np.ascontiguousarray(Dx, dtype=np.complex128)
+    gamma = Dx.ndim == 2
+    e = np.ascontiguousarray(fixed_edge_reps, dtype=np.int64)
+    n_out = 2 * int(batch_max) + int(cutoff) + 1 + (1 if odd_variant else 0)
+    nD = Dx.shape[0] if gamma else 1
+    if batch_max < 0:
+        z = np.zeros(max(n_out, 0), dtype=np.complex128)
+        return np.zeros((nD, z.size), dtype=np.complex128) if gamma else z
+    out = np.zeros(nD * n_out, dtype=np.complex128)
+    args = (e.ctypes.data_as(_ip), e.shape[0], int(batch_max), int(cutoff), int(bool(odd_variant)),
+            int(bool(glynn)), out.ctypes.data_as(_dp), out.shape[0])
+    if gamma:
+        rc = L.wh_loop_hafnian_batch_gamma(_dptr(Ax), _dptr(Dx), nD, Ax.shape[0], *args)
+    else:
+        rc = L.wh_loop_hafnian_batch(_dptr(Ax), _dptr(Dx), Ax.shape[0], *args)
+    if rc < 0:
+        check(rc)
+    return out.reshape(nD, n_out) if gamma else out

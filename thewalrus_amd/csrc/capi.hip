@@ -543,4 +543,99 @@ int wh_calc_loop_hafnian(const double* A, const double* D, const int64_t* edge_r
     return calc_one(A, D, edge_reps, E, oddloop, oddV, glynn, out);
 }
 
+// Shared body of wh_loop_hafnian_batch (nD = 1) and wh_loop_hafnian_batch_gamma.
+static int lhb_call(const double* A, const double* D, int64_t nD, int64_t twoE, const int64_t* fixed_edge_reps,
+                    int64_t Ef, int64_t batch_max, int64_t cutoff, int odd_variant, int glynn, double* out,
+                    int64_t out_len) {
+    if (!A || !D || !out || (Ef > 0 && !fixed_edge_reps)) return fail(WH_EINVAL, "null argument");
+    if (Ef < 0 || batch_max < 0) return fail(WH_EINVAL, "negative size");
+    if (nD < 1) return fail(WH_EINVAL, "no diagonal given");
+    if (cutoff != 0 && cutoff != 1) return fail(WH_EINVAL, "cutoff must be 0 or 1");
+    const int64_t lead = odd_variant ? 2 : 1;
+    const int64_t E = Ef + lead;
+    if (twoE != 2 * E)
+        return fail(WH_EINVAL, "matrix side %lld != 2(Ef + %lld) (Ef=%lld)", (long long)twoE, (long long)lead,
+                    (long long)Ef);
+    if (twoE > kGenericMaxSide)
+        return fail(WH_EUNSUPPORTED, "matrix side %lld exceeds %d", (long long)twoE, kGenericMaxSide);
+    const int64_t nL = 2 * batch_max + cutoff + 1 + (odd_variant ? 1 : 0);
+    if (nL * nD > 128)
+        return fail(WH_EUNSUPPORTED, "%lld outputs (%lld diagonals x %lld) exceed 128", (long long)(nL * nD),
+                    (long long)nD, (long long)nL);
+    wh_lhb_job j;
+    memset(&j, 0, sizeof j);
+    j.e[0] = (int32_t)batch_max;
+    if (odd_variant) j.e[1] = 1;
+    int64_t fsum = 0;
+    double steps_d = (batch_max + 1.0) * (odd_variant ? 2.0 : 1.0);
+    for (int64_t i = 0; i < Ef; ++i) {
+        if (fixed_edge_reps[i] < 0 || fixed_edge_reps[i] > WH_MAX_PHOTONS)
+            return fail(WH_EINVAL, "edge multiplicity %lld out of range", (long long)fixed_edge_reps[i]);
+        j.e[lead + i] = (int32_t)fixed_edge_reps[i];
+        fsum += fixed_edge_reps[i];
+        steps_d *= fixed_edge_reps[i] + 1.0;
+    }
+    const int64_t n_fixed = 2 * fsum + (odd_variant ? 1 : 0);
+    if (n_fixed + nL - 1 > WH_MAX_PHOTONS)
+        return fail(WH_EUNSUPPORTED, "photon number %lld exceeds %d", (long long)(n_fixed + nL - 1), WH_MAX_PHOTONS);
+    if (steps_d > 9.0e18) return fail(WH_EUNSUPPORTED, "term count exceeds int64");
+    if (out_len < nL * nD)
+        return fail(WH_EINVAL, "output holds %lld values, %lld needed", (long long)out_len, (long long)(nL * nD));
+    int64_t steps = (batch_max + 1) * (odd_variant ? 2 : 1);
+    for (int64_t i = 0; i < Ef; ++i) steps *= fixed_edge_reps[i] + 1;
+    int64_t chunk = std::min<int64_t>(std::max<int64_t>(steps / 1024, 1), 256);
+    j.E = (int32_t)E;
+    j.L = (int32_t)nL;
+    j.nD = (int32_t)nD;
+    j.n_fixed = (int32_t)n_fixed;
+    j.n_max = (int32_t)(n_fixed + nL - 1);
+    j.flags = (odd_variant ? WH_LHB_ODD : 0) | (glynn ? WH_LHB_GLYNN : 0);
+    j.steps = steps;
+    j.chunk_terms = chunk;
+    j.nchunks = (int32_t)((steps + chunk - 1) / chunk);
+
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    Context* c;
+    rc = ensure_context(dev, &c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    // staging (bytes): [job | A | D rows | outputs]; the output area is written on the device
+    // and copied back into the same offsets of the pinned block
+    const size_t nA = (size_t)twoE * twoE, nDv = (size_t)nD * twoE, nout = (size_t)(nL * nD);
+    const size_t job_bytes = (sizeof(wh_lhb_job) + 255) & ~(size_t)255;
+    j.a_off = 0;
+    j.d_off = (int64_t)nA;
+    const size_t in_bytes = job_bytes + (nA + nDv) * 16;
+    const size_t out_at = (in_bytes + 255) & ~(size_t)255;
+    rc = ensure_stage(*c, out_at + nout * 16);
+    if (rc) return rc;
+    rc = ensure_partials(*c, (size_t)j.nchunks * nout);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(c->h_stage, &j, sizeof j);
+    memcpy(c->h_stage + job_bytes, A, nA * 16);
+    memcpy(c->h_stage + job_bytes + nA * 16, D, nDv * 16);
+    HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, in_bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(wh_launch_lhb((const wh_lhb_job*)c->d_stage, 1, c->d_stage + job_bytes, c->d_partials, j.nchunks,
+                          (int)twoE, c->d_stage + out_at, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_stage + out_at, c->d_stage + out_at, nout * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(out, c->h_stage + out_at, nout * 16);
+    return (int)(nL * nD);
+}
+
+int wh_loop_hafnian_batch(const double* A, const double* D, int64_t twoE, const int64_t* fixed_edge_reps,
+                          int64_t Ef, int64_t batch_max, int64_t cutoff, int odd_variant, int glynn,
+                          double* out, int64_t out_len) {
+    return lhb_call(A, D, 1, twoE, fixed_edge_reps, Ef, batch_max, cutoff, odd_variant, glynn, out, out_len);
+}
+
+int wh_loop_hafnian_batch_gamma(const double* A, const double* D, int64_t nD, int64_t twoE,
+                                const int64_t* fixed_edge_reps, int64_t Ef, int64_t batch_max, int64_t cutoff,
+                                int odd_variant, int glynn, double* out, int64_t out_len) {
+    return lhb_call(A, D, nD, twoE, fixed_edge_reps, Ef, batch_max, cutoff, odd_variant, glynn, out, out_len);
+}
+
 }  // extern "C"

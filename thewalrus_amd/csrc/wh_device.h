@@ -31,6 +31,31 @@ struct __attribute__((aligned(16))) wh_job_dev {
 #define WH_F_GLYNN 4
 #define WH_F_REAL 8
 
+// One loop_hafnian_batch request (thewalrus/loop_hafnian_batch.py:52, :128; with nD > 1
+// loop_hafnian_batch_gamma.py:53, :132): the reordered matrix and nD diagonals, edge
+// multiplicities with the batched edge first (and the odd-mode edge second for the odd
+// variant), and nD x L outputs, one per diagonal and photon number of the batched mode.
+struct __attribute__((aligned(16))) wh_lhb_job {
+    int64_t a_off;        // element offset of Ax (2E x 2E) in the complex pool
+    int64_t d_off;        // element offset of Dx (nD rows of 2E)
+    int64_t steps;        // prod(e_i + 1) terms (no Glynn halving in the batched sieve)
+    int64_t chunk_base;   // index of this job's first workgroup
+    int64_t chunk_terms;  // terms per workgroup
+    int64_t part_off;     // first partial slot (nchunks x L slots)
+    int64_t out_off;      // first output element
+    int32_t E, L;         // edges, outputs
+    int32_t n_fixed;      // photons fixed by the other modes (2 sum(fixed) [+1 odd variant])
+    int32_t n_max;        // n_fixed + L - 1
+    int32_t flags;        // WH_LHB_* bits
+    int32_t nchunks;
+    int32_t nD;           // diagonals sharing Ax (loop_hafnian_batch_gamma: one per fanout row)
+    int32_t pad_;
+    int32_t e[WH_EMAX];
+};
+
+#define WH_LHB_ODD 1
+#define WH_LHB_GLYNN 2
+
 struct cplx {
     double re, im;
 };
@@ -136,14 +161,19 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ cplx wave_csum(cplx z) { return cmk(wave_sum(z.re), wave_sum(z.im)); }
 
 // Binomial coefficient as the reference's table holds it (_hafnian.py:27-45), in double.
+// Binomial coefficient, exact in 64-bit: each step divides out the common factor first, so no
+// intermediate exceeds the result (C(64, 32) < 2^64 covers every multiplicity up to WH_NMAX / 2).
 __device__ __forceinline__ double binom_d(int n, int k) {
     if (k < 0 || k > n) return 0.0;
     if (k == 0 || k == n) return 1.0;
     int kk = k < n - k ? k : n - k;
-    int64_t b = 1;
+    uint64_t b = 1;
     for (int i = 0; i < kk; ++i) {
-        b *= (n - i);
-        b /= (i + 1);
+        uint64_t num = (uint64_t)(n - i), den = (uint64_t)(i + 1);
+        uint64_t x = b, y = den;
+        while (y) { uint64_t r = x % y; x = y; y = r; }
+        b /= x;
+        b *= num / (den / x);
     }
     return (double)b;
 }

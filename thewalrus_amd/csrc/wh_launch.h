@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 struct wh_job_dev;
+struct wh_lhb_job;
 
 size_t wh_generic_lds_bytes(int MM);
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
@@ -15,3 +16,6 @@ hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_de
                          int nw, hipStream_t stream);
 hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partials, void* out,
                           hipStream_t stream);
+size_t wh_lhb_lds_bytes(int MM);
+hipError_t wh_launch_lhb(const wh_lhb_job* jobs, int njobs, const void* pool, void* partials,
+                         int64_t nblocks, int MM, void* out, hipStream_t stream);
