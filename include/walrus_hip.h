@@ -77,6 +77,14 @@ int wh_loop_hafnian_batch_gamma(const double* A, const double* D, int64_t nD, in
                                 const int64_t* fixed_edge_reps, int64_t Ef, int64_t batch_max, int64_t cutoff,
                                 int odd_variant, int glynn, double* out, int64_t out_len);
 
+/* Replaces montrealer(Sigma) (thewalrus/_montrealer.py:37-57): Sigma = X A, 2n x 2n
+ * complex128 with n <= 24 modes; out: 2 doubles.  Synchronous. */
+int wh_montrealer(const double* Sigma, int64_t twoN, double* out);
+
+/* Replaces lmontrealer(Sigma, zeta) (thewalrus/_montrealer.py:79-101): zeta, 2n complex128
+ * displacements.  Synchronous. */
+int wh_lmontrealer(const double* Sigma, const double* zeta, int64_t twoN, double* out);
+
 /* One prepared request for the batched path.  Offsets count complex128 elements from the
  * start of a device-resident pool that holds every request's reordered matrix (and vectors). */
 typedef struct wh_request {
@@ -98,6 +106,11 @@ typedef struct wh_request {
  * reference does (h - 1 products per term) instead of the default half-products formulation
  * (powers up to B^ceil(h/2), traces of higher powers as <B^a, (B^b)^T>). */
 #define WH_OPT_FULL 2
+/* wh_request.options: a montrealer request (montrealer / lmontrealer, _montrealer.py:37-101)
+ * over n = E modes.  a_off: X Sigma (2E x 2E, rows of Sigma swapped by halves); for the loop
+ * montrealer d_off: X zeta and v_off: conj(X zeta), else both -1.  edge_reps, glynn and
+ * oddloop are ignored.  E <= 24 (the all-ones kernel's tile limit). */
+#define WH_OPT_MONTREALER 4
 
 /* ---- Batches over caller-owned device memory (no allocation, no host synchronisation).
  * A batch is the host-side layout of n requests: kernel choice per request, term chunks and

@@ -42,6 +42,7 @@ def lib():
         L.wo_fcoeffs.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         L.wo_brute_force.argtypes = [dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
         L.wo_kan.argtypes = [dp, dp, ip, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
+        L.wo_montrealer.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
         L.wo_loop_hafnian_batch.argtypes = [dp, dp, ctypes.c_int64, ip, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         _LIB = L
@@ -314,3 +315,61 @@ def loop_hafnian_batch_gamma(A, D, fixed_reps, N_cutoff, glynn=True, precision="
     shares each term's matrix between rows; every row's arithmetic is that of the batch)."""
     D = np.atleast_2d(np.asarray(D))
     return np.array([loop_hafnian_batch(A, D[k], fixed_reps, N_cutoff, glynn, precision) for k in range(D.shape[0])])
+
+
+# --------------------------------------------------------------------------------------
+# montrealer (_montrealer.py)
+# --------------------------------------------------------------------------------------
+def xmat(n):
+    """[[0, I_n], [I_n, 0]] (quantum/conversions.py:55-67)."""
+    I = np.identity(n)
+    O_ = np.zeros((n, n))
+    return np.block([[O_, I], [I, O_]])
+
+
+def montrealer(Sigma, zeta=None, precision="double", nthreads=0):
+    """montrealer(Sigma) / lmontrealer(Sigma, zeta) (_montrealer.py:37-101)."""
+    S = _c(Sigma)
+    z = None if zeta is None else _c(zeta)
+    out = np.zeros(4, dtype=np.float64)
+    p = _prec(precision)
+    rc = lib().wo_montrealer(_dp(S), _dp(z), S.shape[0], p, nthreads, _dp(out))
+    if rc != 0:
+        raise ValueError("montrealer: bad size %d" % S.shape[0])
+    return _unpack(out, p)
+
+
+def mtl(A, zeta=None, precision="double"):
+    """mtl(A) / lmtl(A, zeta): Sigma = X A (_montrealer.py:104-135)."""
+    n = len(A) // 2
+    return montrealer(xmat(n) @ np.asarray(A), zeta, precision)
+
+
+def mtl_walks(A, zeta=None):
+    """Montrealer by enumeration of closed walks (pure Python, n <= 6 modes).
+
+    tr(Sigma_S^n) summed with the inclusion-exclusion signs keeps the closed walks of length n
+    on the 2n vertices that visit every mode once; a step from vertex u to vertex v weighs
+    Sigma[u, v] = A[u ^ n, v] (u ^ n: the other copy of u's mode).  Each walk is counted once
+    per starting point and direction, hence mtl = sum / (2n).  The loop part adds the open
+    walks visiting every mode, conj(zeta[v_0]) prod A[v_k ^ n, v_(k+1)] zeta[v_(n-1)], over 2.
+    """
+    from itertools import permutations, product
+
+    A = np.asarray(A)
+    n = len(A) // 2
+    other = lambda v: v + n if v < n else v - n
+    closed, opened = 0j, 0j
+    for order in permutations(range(n)):
+        for copies in product((0, 1), repeat=n):
+            v = [order[k] + n * copies[k] for k in range(n)]
+            w = 1 + 0j
+            for k in range(n - 1):
+                w *= A[other(v[k]), v[k + 1]]
+            closed += w * A[other(v[n - 1]), v[0]]
+            if zeta is not None:
+                opened += np.conj(zeta[v[0]]) * w * zeta[v[n - 1]]
+    val = closed / (2 * n)
+    if zeta is not None:
+        val += opened / 2
+    return val

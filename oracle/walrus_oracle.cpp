@@ -842,3 +842,71 @@ extern "C" int wo_loop_hafnian_batch(const double* A, const double* D, int64_t t
     for (size_t q = 0; q < H.size(); ++q) store(H[q], out + 4 * q);
     return (int)H.size();
 }
+
+// ---------------------------------------------------------------------------------------
+// _montrealer.py restatement (montrealer / lmontrealer over Sigma = X A)
+// ---------------------------------------------------------------------------------------
+
+// _montrealer.py:37-57 (zeta == nullptr) and :79-101: the inclusion-exclusion over subsets S
+// of the n modes (dec2bin order, :17-33: bit n-1-i of p selects mode i), rows/columns
+// S u (S + n) of Sigma, sign (-1)^(|S|+1), p_n = powertrace(sub, n + 1)[-1] and, with zeta,
+// power_loop(sub, zeta_S, n) = conj(zeta_S) sub^(n-1) zeta_S (:60-76), taken before
+// powertrace (which may reduce sub in place).  Result (-1)^(n+1) (val / 2n + loops / 2).
+template <class T>
+static cx<T> montrealer(const cvec<T>& Sg, const cvec<T>* zeta, int n2, int nthreads) {
+    const int n = n2 / 2;
+    const int64_t tot = (int64_t)1 << n;
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    std::vector<cx<T>> pv(nt), pl(nt);
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = omp_get_thread_num();
+        cx<T> val, loops;
+        std::vector<int> pos;
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < tot; ++p) {
+            pos.clear();
+            for (int i = 0; i < n; ++i)
+                if ((p >> (n - 1 - i)) & 1) pos.push_back(i);
+            const int len = (int)pos.size();
+            for (int i = 0; i < len; ++i) pos.push_back(pos[i] + n);
+            const int m = 2 * len;
+            cvec<T> sub((size_t)m * m);
+            for (int r = 0; r < m; ++r)
+                for (int c = 0; c < m; ++c) sub[(size_t)r * m + c] = Sg[(size_t)pos[r] * n2 + pos[c]];
+            const T sign = T(((len + 1) & 1) ? -1.0 : 1.0);
+            if (zeta) {
+                cvec<T> sv(m), vec;
+                for (int r = 0; r < m; ++r) sv[r] = (*zeta)[pos[r]];
+                vec = sv;
+                for (int it = 0; it < n - 1; ++it) vec = matvec(sub, vec, m);
+                cvec<T> cz(m);
+                for (int r = 0; r < m; ++r) cz[r] = conj(sv[r]);
+                loops += mulr(dotu(cz, vec), sign);
+            }
+            cvec<T> pt = powertrace(sub, m, n + 1);
+            val += mulr(pt[n], sign);
+        }
+        pv[tid] = val;
+        pl[tid] = loops;
+    }
+    cx<T> val, loops;
+    for (int t = 0; t < nt; ++t) { val += pv[t]; loops += pl[t]; }
+    const T s = T((n + 1) & 1 ? -1.0 : 1.0);
+    cx<T> r = divr(val, T((double)(2 * n)));
+    if (zeta) r += divr(loops, T(2.0));
+    return mulr(r, s);
+}
+
+extern "C" int wo_montrealer(const double* Sigma, const double* zeta, int64_t n2, int precision, int nthreads,
+                             double* out) {
+    if (n2 % 2 || n2 < 2 || n2 > 64) return -1;
+    if (precision == 0) {
+        cvec<double> z = zeta ? load<double>(zeta, n2) : cvec<double>();
+        store(montrealer(load<double>(Sigma, (size_t)n2 * n2), zeta ? &z : nullptr, (int)n2, nthreads), out);
+        return 0;
+    }
+    cvec<dd> z = zeta ? load<dd>(zeta, n2) : cvec<dd>();
+    store(montrealer(load<dd>(Sigma, (size_t)n2 * n2), zeta ? &z : nullptr, (int)n2, nthreads), out);
+    return 0;
+}

@@ -56,6 +56,8 @@ EXPORTS = [
     "wh_plan_destroy",
     "wh_loop_hafnian_batch",
     "wh_loop_hafnian_batch_gamma",
+    "wh_montrealer",
+    "wh_lmontrealer",
 ]
 
 _lib = None
@@ -98,6 +100,8 @@ def load(path=None):
         L.wh_plan_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         L.wh_loop_hafnian_batch.argtypes = [_dp, _dp, ctypes.c_int64, _ip, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64]
+        L.wh_montrealer.argtypes = [_dp, ctypes.c_int64, _dp]
+        L.wh_lmontrealer.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
         L.wh_loop_hafnian_batch_gamma.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _ip, ctypes.c_int64,
                                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp,
                                                   ctypes.c_int64]
@@ -174,3 +178,17 @@ def loop_hafnian_batch(Ax, Dx, fixed_edge_reps, batch_max, cutoff, odd_variant, 
     if rc < 0:
         check(rc)
     return out.reshape(nD, n_out) if gamma else out
+
+
+def montrealer(Sigma, zeta=None):
+    """Device sieve at the reference seams ``montrealer(Sigma)`` / ``lmontrealer(Sigma, zeta)``
+    (``_montrealer.py:37``, ``:79``)."""
+    L = load()
+    S = np.ascontiguousarray(Sigma, dtype=np.complex128)
+    out = np.zeros(2, dtype=np.float64)
+    if zeta is None:
+        check(L.wh_montrealer(_dptr(S), S.shape[0], _dptr(out)))
+    else:
+        z = np.ascontiguousarray(zeta, dtype=np.complex128)
+        check(L.wh_lmontrealer(_dptr(S), _dptr(z), S.shape[0], _dptr(out)))
+    return np.complex128(complex(out[0], out[1]))

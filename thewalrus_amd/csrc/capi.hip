@@ -41,6 +41,7 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 constexpr int kGenericMaxSide = 48;  // LDS holds three 48 x 48 complex matrices per wave
+constexpr int kMtlMaxModes = 24;     // montrealer on the all-ones kernel: side 2n <= 8 * 6
 
 // Per-device state used by the synchronous per-call entry points.
 struct Context {
@@ -111,6 +112,22 @@ int ensure_partials(Context& c, size_t n) {
 int describe(const wh_request& r, wh_job_dev& j) {
     if (r.E < 1 || r.E > WH_MAX_EDGES) return fail(WH_EINVAL, "E=%d outside 1..%d", r.E, WH_MAX_EDGES);
     memset(&j, 0, sizeof j);
+    if (r.options & WH_OPT_MONTREALER) {
+        // _montrealer.py:48-57: all 2^n subsets of the n = E modes, p_n of a 2n-side matrix
+        if (r.E > kMtlMaxModes)
+            return fail(WH_EUNSUPPORTED, "montrealer of %d modes exceeds %d", r.E, kMtlMaxModes);
+        const bool loop = r.d_off >= 0;
+        if (loop != (r.v_off >= 0)) return fail(WH_EINVAL, "loop montrealer needs both X zeta and its conjugate");
+        for (int i = 0; i < r.E; ++i) j.e[i] = 1;
+        j.a_off = r.a_off;
+        j.d_off = r.d_off;
+        j.v_off = r.v_off;
+        j.steps = (int64_t)1 << r.E;
+        j.E = r.E;
+        j.N = 2 * r.E;
+        j.flags = WH_F_MTL | (loop ? WH_F_LOOP : 0) | (r.real ? WH_F_REAL : 0);
+        return WH_OK;
+    }
     int64_t esum = 0;
     double steps_d = 1.0;
     const bool odd = r.v_off >= 0;
@@ -169,6 +186,14 @@ struct GroupKey {
 
 GroupKey classify(const wh_job_dev& j, int options) {
     GroupKey g{KIND_GENERIC, 0, 0, 0, 0};
+    if (j.flags & WH_F_MTL) {  // the montrealer runs on the all-ones kernel only
+        g.kind = KIND_PM1;
+        g.T = (2 * j.E + 7) / 8;
+        g.real = (j.flags & WH_F_REAL) ? 1 : 0;
+        g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
+        g.half = (options & WH_OPT_FULL) ? 0 : 1;
+        return g;
+    }
     if (options & WH_OPT_GENERIC) return g;
     bool all_one = true;
     for (int i = 0; i < j.E; ++i) all_one = all_one && j.e[i] == 1;
@@ -465,14 +490,10 @@ int wh_plan_destroy(wh_plan* p) {
     return WH_OK;
 }
 
-// Shared body of the two synchronous per-call entry points.
-static int calc_one(const double* A, const double* D, const int64_t* edge_reps, int64_t E,
-                    const double* oddloop, const double* oddV, int glynn, double* out) {
-    if (!A || !edge_reps || !out) return fail(WH_EINVAL, "null argument");
-    if (E < 1 || E > WH_MAX_EDGES) return fail(WH_EINVAL, "E=%lld outside 1..%d", (long long)E, WH_MAX_EDGES);
-    if (oddV && !oddloop) return fail(WH_EINVAL, "oddV given without oddloop");
-    if (oddloop && !oddV) return fail(WH_EINVAL, "oddloop given without oddV");
-    if (oddloop && !D) return fail(WH_EINVAL, "odd-N case needs D");
+// Stages one request ([job | A | D | V] in one pinned block, one H2D copy), runs it and
+// waits for the value.  r: a_off / d_off / v_off are filled here from which of D / V is given.
+static int run_request(wh_request r, const double* A, const double* D, const double* V, int64_t side,
+                       double* out) {
     int dev;
     int rc = current_device(&dev);
     if (rc) return rc;
@@ -480,35 +501,16 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
     rc = ensure_context(dev, &c);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-
-    const int64_t side = 2 * E;
-    wh_request r;
-    memset(&r, 0, sizeof r);
-    r.E = (int32_t)E;
-    r.glynn = glynn ? 1 : 0;
-    for (int64_t i = 0; i < E; ++i) {
-        if (edge_reps[i] < 0 || edge_reps[i] > (1 << 20)) return fail(WH_EINVAL, "edge multiplicity out of range");
-        r.edge_reps[i] = (int32_t)edge_reps[i];
-    }
-    {
-        // zero imaginary parts everywhere: the real-arithmetic kernels compute the same values
-        bool real = true;
-        for (size_t i = 0; i < (size_t)side * side && real; ++i) real = A[2 * i + 1] == 0.0;
-        for (int64_t i = 0; D && i < side && real; ++i) real = D[2 * i + 1] == 0.0;
-        if (oddV || oddloop) real = false;
-        r.real = real ? 1 : 0;
-    }
-    // staging layout (complex elements): A at 0, D after it, oddV after that
+    // staging layout (complex elements): A at 0, D after it, V after that
     size_t nA = (size_t)side * side;
     r.a_off = 0;
     r.d_off = D ? (int64_t)nA : -1;
-    r.v_off = oddV ? (int64_t)(nA + side) : -1;
-    if (oddloop) { r.oddloop[0] = oddloop[0]; r.oddloop[1] = oddloop[1]; }
+    r.v_off = V ? (int64_t)(nA + side) : -1;
     Layout L;
     rc = build_layout(&r, 1, 0, L);
     if (rc) return rc;
     size_t job_bytes = (sizeof(wh_job_dev) + 255) & ~(size_t)255;
-    size_t data_elems = nA + (D ? side : 0) + (oddV ? side : 0);
+    size_t data_elems = nA + (D ? side : 0) + (V ? side : 0);
     size_t bytes = job_bytes + data_elems * 16;
     rc = ensure_stage(*c, bytes);
     if (rc) return rc;
@@ -519,7 +521,7 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
     double* hd = (double*)(c->h_stage + job_bytes);
     memcpy(hd, A, nA * 16);
     if (D) memcpy(hd + 2 * nA, D, side * 16);
-    if (oddV) memcpy(hd + 2 * (nA + side), oddV, side * 16);
+    if (V) memcpy(hd + 2 * (nA + side), V, side * 16);
     HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
     const wh_job_dev* djob = (const wh_job_dev*)c->d_stage;
     const void* dpool = c->d_stage + job_bytes;
@@ -529,6 +531,65 @@ static int calc_one(const double* A, const double* D, const int64_t* edge_reps, 
     out[0] = c->h_out[0];
     out[1] = c->h_out[1];
     return WH_OK;
+}
+
+static bool all_real(const double* x, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (x[2 * i + 1] != 0.0) return false;
+    return true;
+}
+
+// Shared body of the two synchronous hafnian entry points.
+static int calc_one(const double* A, const double* D, const int64_t* edge_reps, int64_t E,
+                    const double* oddloop, const double* oddV, int glynn, double* out) {
+    if (!A || !edge_reps || !out) return fail(WH_EINVAL, "null argument");
+    if (E < 1 || E > WH_MAX_EDGES) return fail(WH_EINVAL, "E=%lld outside 1..%d", (long long)E, WH_MAX_EDGES);
+    if (oddV && !oddloop) return fail(WH_EINVAL, "oddV given without oddloop");
+    if (oddloop && !oddV) return fail(WH_EINVAL, "oddloop given without oddV");
+    if (oddloop && !D) return fail(WH_EINVAL, "odd-N case needs D");
+    const int64_t side = 2 * E;
+    wh_request r;
+    memset(&r, 0, sizeof r);
+    r.E = (int32_t)E;
+    r.glynn = glynn ? 1 : 0;
+    for (int64_t i = 0; i < E; ++i) {
+        if (edge_reps[i] < 0 || edge_reps[i] > (1 << 20)) return fail(WH_EINVAL, "edge multiplicity out of range");
+        r.edge_reps[i] = (int32_t)edge_reps[i];
+    }
+    // zero imaginary parts everywhere: the real-arithmetic kernels compute the same values
+    r.real = (all_real(A, (size_t)side * side) && (!D || all_real(D, side)) && !oddV && !oddloop) ? 1 : 0;
+    if (oddloop) { r.oddloop[0] = oddloop[0]; r.oddloop[1] = oddloop[1]; }
+    return run_request(r, A, D, oddV, side, out);
+}
+
+// Shared body of the montrealer entry points: stages X Sigma and, for the loop montrealer,
+// X zeta and conj(X zeta) (the all-ones kernel's M = Ax X then equals X Sigma X).
+static int calc_mtl(const double* Sigma, const double* zeta, int64_t twoN, double* out) {
+    if (!Sigma || !out) return fail(WH_EINVAL, "null argument");
+    if (twoN < 2 || twoN % 2) return fail(WH_EINVAL, "matrix side %lld must be even and >= 2", (long long)twoN);
+    const int64_t n = twoN / 2;
+    if (n > kMtlMaxModes) return fail(WH_EUNSUPPORTED, "montrealer of %lld modes exceeds %d", (long long)n, kMtlMaxModes);
+    std::vector<double> ax((size_t)twoN * twoN * 2), xz, cxz;
+    for (int64_t r = 0; r < twoN; ++r) {
+        int64_t src = r < n ? r + n : r - n;
+        memcpy(&ax[(size_t)r * twoN * 2], Sigma + (size_t)src * twoN * 2, (size_t)twoN * 16);
+    }
+    if (zeta) {
+        xz.resize(2 * twoN);
+        cxz.resize(2 * twoN);
+        for (int64_t e = 0; e < twoN; ++e) {
+            int64_t src = e < n ? e + n : e - n;
+            xz[2 * e] = cxz[2 * e] = zeta[2 * src];
+            xz[2 * e + 1] = zeta[2 * src + 1];
+            cxz[2 * e + 1] = -zeta[2 * src + 1];
+        }
+    }
+    wh_request r;
+    memset(&r, 0, sizeof r);
+    r.E = (int32_t)n;
+    r.options = WH_OPT_MONTREALER;
+    r.real = (all_real(Sigma, (size_t)twoN * twoN) && (!zeta || all_real(zeta, twoN))) ? 1 : 0;
+    return run_request(r, ax.data(), zeta ? xz.data() : nullptr, zeta ? cxz.data() : nullptr, twoN, out);
 }
 
 int wh_calc_hafnian(const double* A, int64_t twoE, const int64_t* edge_reps, int64_t E, int glynn,
@@ -541,6 +602,13 @@ int wh_calc_loop_hafnian(const double* A, const double* D, const int64_t* edge_r
                          const double* oddloop, const double* oddV, int glynn, double* out) {
     if (!D) return fail(WH_EINVAL, "loop hafnian needs D");
     return calc_one(A, D, edge_reps, E, oddloop, oddV, glynn, out);
+}
+
+int wh_montrealer(const double* Sigma, int64_t twoN, double* out) { return calc_mtl(Sigma, nullptr, twoN, out); }
+
+int wh_lmontrealer(const double* Sigma, const double* zeta, int64_t twoN, double* out) {
+    if (!zeta) return fail(WH_EINVAL, "loop montrealer needs zeta");
+    return calc_mtl(Sigma, zeta, twoN, out);
 }
 
 // Shared body of wh_loop_hafnian_batch (nD = 1) and wh_loop_hafnian_batch_gamma.

@@ -205,6 +205,10 @@ extern "C" __global__ void wh_fold(const wh_job_dev* __restrict__ jobs, int njob
         acc.add_pair(p.x, p.y, p.z, p.w);
     }
     double re = acc.sr + acc.cr, im = acc.si + acc.ci;
+    if ((J.flags & WH_F_MTL) && (J.E & 1) == 0) {  // (-1)^(n+1), _montrealer.py:57, :101
+        re = -re;
+        im = -im;
+    }
     if (J.flags & WH_F_GLYNN) {
         int ex = (J.flags & WH_F_ODD) ? J.N / 2 : J.N / 2 - 1;
         double sc = ldexp(1.0, -ex);

@@ -45,11 +45,12 @@ __device__ __forceinline__ cplx eadd(cplx a, cplx b) { return cadd(a, b); }
 __device__ __forceinline__ double ediv(double a, double d) { return a / d; }
 __device__ __forceinline__ cplx ediv(cplx a, double d) { return cdivr(a, d); }
 
-// sign of column c of the current term: delta_(c mod h) = +1 if digit set, else -1
-__device__ __forceinline__ double col_sign(uint64_t t, int c, int h) {
+// factor of column c of the current term: delta_(c mod h) = +1 if digit set, else `unset`
+// (-1 for the Glynn sieve, 0 for the montrealer's subset inclusion-exclusion)
+__device__ __forceinline__ double col_sign(uint64_t t, int c, int h, double unset) {
     int i = c < h ? c : c - h;
     if (i >= h) return 1.0;  // zero padding
-    return ((t >> (h - 1 - i)) & 1ull) ? 1.0 : -1.0;
+    return ((t >> (h - 1 - i)) & 1ull) ? 1.0 : unset;
 }
 
 
@@ -221,6 +222,9 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
     const int ti = lane >> 3, tj = lane & 7;
     const int n = 2 * J.E;
     const int h = J.E;
+    // montrealer terms (_montrealer.py:37-101): 0/1 column factors, p_h and v_h only
+    const bool mtl = (J.flags & WH_F_MTL) != 0;
+    const double unset = mtl ? 0.0 : -1.0;
 
     // ---- LDS carve
     E* Ms = reinterpret_cast<E*>(smem);
@@ -255,9 +259,14 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             ezero(xd);
             ezero(dd);
             if (e < n) {
-                int src = e < h ? e + h : e - h;  // (X D)[e]
-                xd = eload(Dg, src, (E*)nullptr);
-                dd = eload(Dg, e, (E*)nullptr);
+                if (mtl) {  // the caller stages conj(X zeta) and X zeta
+                    xd = eload(pool + J.v_off, e, (E*)nullptr);
+                    dd = eload(Dg, e, (E*)nullptr);
+                } else {
+                    int src = e < h ? e + h : e - h;  // (X D)[e]
+                    xd = eload(Dg, src, (E*)nullptr);
+                    dd = eload(Dg, e, (E*)nullptr);
+                }
             }
             XD0[e] = xd;
             Dv[e] = dd;
@@ -277,8 +286,9 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
         const uint64_t tu = (uint64_t)t;
         double sgnJ[T];
 #pragma unroll
-        for (int c = 0; c < T; ++c) sgnJ[c] = col_sign(tu, J0 + c, h);
-        const double prefac = ((h - __popcll(tu)) & 1) ? -1.0 : 1.0;
+        for (int c = 0; c < T; ++c) sgnJ[c] = col_sign(tu, J0 + c, h, unset);
+        const double prefac = mtl ? (((__popcll(tu) + 1) & 1) ? -1.0 : 1.0)
+                                  : (((h - __popcll(tu)) & 1) ? -1.0 : 1.0);
 
         // P_1 = M diag(d)
         E acc[T][T];
@@ -291,7 +301,7 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
         // used so they do not occupy registers across the products
         auto load_xd = [&](E (&xdI)[T]) {
 #pragma unroll
-            for (int r = 0; r < T; ++r) xdI[r] = escale(XD0[I0 + r], col_sign(tu, I0 + r, h));
+            for (int r = 0; r < T; ++r) xdI[r] = escale(XD0[I0 + r], col_sign(tu, I0 + r, h, unset));
         };
         auto load_d = [&](E (&dJ)[T]) {
 #pragma unroll
@@ -382,11 +392,20 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             // v_1 = XD_d . D
             E part;
             ezero(part);
-            for (int e = lane; e < n; e += WH_WAVE) emac(part, escale(XD0[e], col_sign(tu, e, h)), Dv[e]);
+            for (int e = lane; e < n; e += WH_WAVE) emac(part, escale(XD0[e], col_sign(tu, e, h, unset)), Dv[e]);
             cplx v1 = wave_csum(etoc(part));
             if (lane == 0) lv[1] = v1;
         }
 
+        if (mtl) {
+            // montrealer term: sign (p_h / 2h + v_h / 2), v_h = conj(zeta_S) Sigma_S^(h-1) zeta_S
+            if (lane == h) {
+                E f = ediv(eget(pt[h], (E*)nullptr), (double)(2 * h));
+                if (LOOP) f = eadd(f, ediv(eget(lv[h], (E*)nullptr), 2.0));
+                acc_sum.add(cmulr(etoc(f), prefac));
+            }
+            continue;
+        }
         // coefficient of eta^h in exp(sum_i a_i eta^i) (_hafnian.py:200-208, :232-241); the
         // lanes hold the coefficients 0..h and add the shifted old row in the reference's j order
         E* oldc = comb0;
