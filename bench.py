@@ -15,6 +15,8 @@ of R requests of that stream with every request's reordered matrix already resid
     per-call dispatch cost matters;
   * configs: the other benchmark configurations (1: n = 20 per call; 2: 4096 real 24 x 24 in
     one launch; 3: n = 40 loop single solve; 4: 1024 hafnian_repeated requests);
+  * next_rows: the GBS samplers' batched loop hafnians (loop_hafnian_batch, _gamma) and the
+    montrealer at 16 and 20 modes, per call, with the CPU oracle beside them;
   * roofline: FP64 product flops of the dominant kernel / its HIP-event time (same stream);
   * cpu_baseline: the CPU oracle (C++ restatement of the reference algorithm, OpenMP over
     terms, all host cores) on a bounded sample of the stream, scaled to hafnians/s by
@@ -150,6 +152,61 @@ def configs_run(torch, tw, B, stream):
     return out
 
 
+def next_rows_run(with_cpu):
+    """The sampler's batched loop hafnian and the montrealer: device time per call (host
+    buffers in and out, as the drop-ins take them) and, with with_cpu, the CPU oracle on the
+    same inputs."""
+    from thewalrus_amd import lmtl, mtl
+    from thewalrus_amd.loop_hafnian_batch import loop_hafnian_batch
+    from thewalrus_amd.loop_hafnian_batch_gamma import loop_hafnian_batch_gamma
+
+    out = {}
+    rng = np.random.default_rng(2024)
+
+    def per_call_ms(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    # hafnian sampler step: 16 modes, 15 detected with 0-3 photons each, cutoff 12 on the last
+    n = 16
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A = (X + X.T) * 0.1
+    D = rng.normal(size=n) + 1j * rng.normal(size=n)
+    fixed = list(rng.integers(0, 4, size=n - 1))
+    row = {"workload": "loop_hafnian_batch, %d modes, fixed photons %d, N_cutoff 12" % (n, sum(fixed)),
+           "gpu_ms": per_call_ms(lambda: loop_hafnian_batch(A, D, fixed, 12), 5)}
+    if with_cpu:
+        from oracle import oracle as O
+
+        row["cpu_oracle_ms"] = per_call_ms(lambda: O.loop_hafnian_batch(A, D, fixed, 12), 1)
+    out["loop_hafnian_batch"] = row
+    # threshold sampler step: fanout 10 diagonals, cutoff 1
+    Dg = rng.normal(size=(10, n)) + 1j * rng.normal(size=(10, n))
+    fixed1 = [int(f > 0) for f in fixed]
+    row = {"workload": "loop_hafnian_batch_gamma, %d modes, %d clicks, fanout 10, N_cutoff 1" % (n, sum(fixed1)),
+           "gpu_ms": per_call_ms(lambda: loop_hafnian_batch_gamma(A, Dg, fixed1, 1), 5)}
+    if with_cpu:
+        row["cpu_oracle_ms"] = per_call_ms(lambda: O.loop_hafnian_batch_gamma(A, Dg, fixed1, 1), 1)
+    out["loop_hafnian_batch_gamma"] = row
+
+    # montrealer: 2^n subsets of 2n x 2n products
+    for nm in (16, 20):
+        S = rng.normal(size=(2 * nm, 2 * nm)) + 1j * rng.normal(size=(2 * nm, 2 * nm))
+        S = (S + S.T) / (2 * np.sqrt(2 * nm))
+        z = rng.normal(size=2 * nm) + 1j * rng.normal(size=2 * nm)
+        flops = 2.0 ** nm * 8 * (2 * nm) ** 3 * ((nm + 1) // 2 - 1)
+        ms = per_call_ms(lambda: mtl(S), 2)
+        row = {"gpu_ms": ms, "gpu_ms_loop": per_call_ms(lambda: lmtl(S, z), 2),
+               "product_tflops": flops / (ms / 1e3) / 1e12}
+        if with_cpu and nm == 16:
+            row["cpu_oracle_ms"] = per_call_ms(lambda: O.mtl(S), 1)
+        out["mtl_n%d" % nm] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +219,7 @@ def main():
     ap.add_argument("--no-percall", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the batched loop hafnian / montrealer lines")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -259,6 +317,7 @@ def main():
         sb.close()
 
     configs = None if args.no_configs else configs_run(torch, tw, B, stream)
+    next_rows = None if args.no_next else next_rows_run(not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(reqs)
     head = serving["load_0.9"] if serving else {}
     line = {
@@ -283,6 +342,7 @@ def main():
         "serving": serving,
         "small_stream": small,
         "configs": configs,
+        "next_rows": next_rows,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
