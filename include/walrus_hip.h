@@ -85,6 +85,14 @@ int wh_montrealer(const double* Sigma, int64_t twoN, double* out);
  * displacements.  Synchronous. */
 int wh_lmontrealer(const double* Sigma, const double* zeta, int64_t twoN, double* out);
 
+/* Replaces _hermite_multidimensional(R, y, G) and _hermite_multidimensional_renorm(R, y, G)
+ * (thewalrus/_hermite_multidimensional.py:279-303, :251-275), the recursion behind
+ * hermite_multidimensional and hafnian_batched (:26, :173).  R: n x n complex128, y: n
+ * complex128, cutoffs: n sizes, C: one complex128 (the value at index 0).  out: the
+ * prod(cutoffs) complex128 entries of G, row-major.  n <= 32.  Synchronous. */
+int wh_hermite_multidimensional(const double* R, const double* y, int64_t n, const int64_t* cutoffs,
+                                const double* C, int renorm, double* out);
+
 /* One prepared request for the batched path.  Offsets count complex128 elements from the
  * start of a device-resident pool that holds every request's reordered matrix (and vectors). */
 typedef struct wh_request {

@@ -42,6 +42,7 @@ def lib():
         L.wo_fcoeffs.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         L.wo_brute_force.argtypes = [dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
         L.wo_kan.argtypes = [dp, dp, ip, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
+        L.wo_hermite_multidimensional.argtypes = [dp, dp, ctypes.c_int64, ip, dp, ctypes.c_int, ctypes.c_int, dp]
         L.wo_montrealer.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, dp]
         L.wo_loop_hafnian_batch.argtypes = [dp, dp, ctypes.c_int64, ip, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
@@ -373,3 +374,30 @@ def mtl_walks(A, zeta=None):
     if zeta is not None:
         val += opened / 2
     return val
+
+
+# --------------------------------------------------------------------------------------
+# hermite_multidimensional / hafnian_batched (_hermite_multidimensional.py)
+# --------------------------------------------------------------------------------------
+def hermite_recursion(R, y, cutoffs, C=1.0, renorm=False, precision="double"):
+    """_hermite_multidimensional(_renorm)(R, y, G) with G[0] = C (:251-303): the full tensor."""
+    R = _c(R)
+    y = _c(y)
+    cut = np.ascontiguousarray(cutoffs, dtype=np.int64)
+    total = int(np.prod(cut))
+    p = _prec(precision)
+    out = np.zeros((4 if p else 2) * total, dtype=np.float64)
+    c = np.array([complex(C).real, complex(C).imag])
+    lib().wo_hermite_multidimensional(_dp(R), _dp(y), R.shape[0], _ip(cut), _dp(c), int(bool(renorm)), p, _dp(out))
+    if p == 0:
+        return out.view(np.complex128).reshape(tuple(cut))
+    o = out.reshape(-1, 4)
+    return ((o[:, 0] + o[:, 1]) + 1j * (o[:, 2] + o[:, 3])).reshape(tuple(cut))
+
+
+def hafnian_batched(A, cutoff, mu=None, renorm=False, precision="double"):
+    """hafnian_batched (:173-216): the recursion with R = -A, y = mu (modified form)."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    mu = np.zeros(n, dtype=complex) if mu is None else np.asarray(mu)
+    return hermite_recursion(-A, mu, [cutoff] * n, 1.0, renorm, precision)

@@ -910,3 +910,55 @@ extern "C" int wo_montrealer(const double* Sigma, const double* zeta, int64_t n2
     store(montrealer(load<dd>(Sigma, (size_t)n2 * n2), zeta ? &z : nullptr, (int)n2, nthreads), out);
     return 0;
 }
+
+// ---------------------------------------------------------------------------------------
+// _hermite_multidimensional.py restatement (hafnian_batched / hermite_multidimensional)
+// ---------------------------------------------------------------------------------------
+
+// _hermite_multidimensional.py:251-275 (renorm) and :279-303: G in row-major (np.ndindex)
+// order, G[0] = C given; for every other index, i = the first nonzero position,
+// ki = idx - e_i, G[idx] = y_i G[ki] - sum_(l: ki_l > 0) w(ki_l) R_il G[ki - e_l] with
+// w(k) = k, or w(k) = sqrt(k) and a final division by sqrt(idx_i) when renormalised.
+template <class T>
+static void hermite_md(const cvec<T>& R, const cvec<T>& y, int n, const int64_t* cut, bool renorm, cvec<T>& G) {
+    std::vector<int64_t> stride(n);
+    int64_t total = 1;
+    for (int l = n - 1; l >= 0; --l) { stride[l] = total; total *= cut[l]; }
+    std::vector<int64_t> idx(n, 0);
+    auto sq = [](int64_t k) { return rsqrt_of(T((double)k)); };  // sqrt, in T's precision
+    for (int64_t lin = 1; lin < total; ++lin) {
+        for (int l = n - 1; l >= 0; --l) {  // advance the multi-index (last position fastest)
+            if (++idx[l] < cut[l]) break;
+            idx[l] = 0;
+        }
+        int i = 0;
+        while (idx[i] == 0) ++i;
+        const int64_t ki = lin - stride[i];
+        cx<T> u = y[i] * G[ki];
+        for (int l = 0; l < n; ++l) {
+            const int64_t kl = idx[l] - (l == i ? 1 : 0);
+            if (kl <= 0) continue;
+            const T w = renorm ? sq(kl) : T((double)kl);
+            u -= (mulr(R[(size_t)i * n + l], w)) * G[ki - stride[l]];
+        }
+        G[lin] = renorm ? divr(u, sq(idx[i])) : u;
+    }
+}
+
+extern "C" int wo_hermite_multidimensional(const double* R, const double* y, int64_t n, const int64_t* cut,
+                                           const double* C, int renorm, int precision, double* out) {
+    int64_t total = 1;
+    for (int64_t l = 0; l < n; ++l) total *= cut[l];
+    if (precision == 0) {
+        cvec<double> G(total);
+        G[0] = cx<double>(C[0], C[1]);
+        hermite_md(load<double>(R, (size_t)n * n), load<double>(y, n), (int)n, cut, renorm != 0, G);
+        for (int64_t q = 0; q < total; ++q) store(G[q], out + 2 * q);
+        return 0;
+    }
+    cvec<dd> G(total);
+    G[0] = cx<dd>(dd(C[0]), dd(C[1]));
+    hermite_md(load<dd>(R, (size_t)n * n), load<dd>(y, n), (int)n, cut, renorm != 0, G);
+    for (int64_t q = 0; q < total; ++q) store(G[q], out + 4 * q);
+    return 0;
+}

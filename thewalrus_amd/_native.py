@@ -58,6 +58,7 @@ EXPORTS = [
     "wh_loop_hafnian_batch_gamma",
     "wh_montrealer",
     "wh_lmontrealer",
+    "wh_hermite_multidimensional",
 ]
 
 _lib = None
@@ -100,6 +101,7 @@ def load(path=None):
         L.wh_plan_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         L.wh_loop_hafnian_batch.argtypes = [_dp, _dp, ctypes.c_int64, _ip, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64]
+        L.wh_hermite_multidimensional.argtypes = [_dp, _dp, ctypes.c_int64, _ip, _dp, ctypes.c_int, _dp]
         L.wh_montrealer.argtypes = [_dp, ctypes.c_int64, _dp]
         L.wh_lmontrealer.argtypes = [_dp, _dp, ctypes.c_int64, _dp]
         L.wh_loop_hafnian_batch_gamma.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _ip, ctypes.c_int64,
@@ -192,3 +194,18 @@ def montrealer(Sigma, zeta=None):
         z = np.ascontiguousarray(zeta, dtype=np.complex128)
         check(L.wh_lmontrealer(_dptr(S), _dptr(z), S.shape[0], _dptr(out)))
     return np.complex128(complex(out[0], out[1]))
+
+
+def hermite_recursion(R, y, cutoffs, C=1.0, renorm=False):
+    """Device recursion at the reference seams ``_hermite_multidimensional(R, y, G)`` /
+    ``_hermite_multidimensional_renorm`` (``_hermite_multidimensional.py:279``, ``:251``) with
+    ``G[0] = C``: the full complex128 tensor of shape ``cutoffs``."""
+    L = load()
+    R = np.ascontiguousarray(R, dtype=np.complex128)
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    cut = np.ascontiguousarray(cutoffs, dtype=np.int64)
+    out = np.empty(tuple(int(c) for c in cut), dtype=np.complex128)
+    c = np.array([complex(C)], dtype=np.complex128)
+    check(L.wh_hermite_multidimensional(_dptr(R), _dptr(y), R.shape[0], cut.ctypes.data_as(_ip), _dptr(c),
+                                        int(bool(renorm)), out.ctypes.data_as(_dp)))
+    return out

@@ -706,4 +706,50 @@ int wh_loop_hafnian_batch_gamma(const double* A, const double* D, int64_t nD, in
     return lhb_call(A, D, nD, twoE, fixed_edge_reps, Ef, batch_max, cutoff, odd_variant, glynn, out, out_len);
 }
 
+int wh_hermite_multidimensional(const double* R, const double* y, int64_t n, const int64_t* cutoffs,
+                                const double* C, int renorm, double* out) {
+    if (!R || !y || !cutoffs || !C || !out) return fail(WH_EINVAL, "null argument");
+    if (n < 1 || n > 32) return fail(WH_EUNSUPPORTED, "n=%lld outside 1..32", (long long)n);
+    int cut[32];
+    int64_t total = 1;
+    int maxcut = 1;
+    for (int64_t l = 0; l < n; ++l) {
+        if (cutoffs[l] < 1 || cutoffs[l] > (1 << 20)) return fail(WH_EINVAL, "cutoff %lld out of range", (long long)cutoffs[l]);
+        cut[l] = (int)cutoffs[l];
+        maxcut = std::max(maxcut, cut[l]);
+        if ((double)total * cut[l] > 4.0e10) return fail(WH_EUNSUPPORTED, "tensor of more than 4e10 entries");
+        total *= cut[l];
+    }
+    int dev;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    Context* c;
+    rc = ensure_context(dev, &c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    // staging (bytes): [R | y | sqrt table]; the tensor gets its own device allocation
+    const size_t nR = (size_t)n * n * 16, ny = (size_t)n * 16, nsq = (size_t)maxcut * 8;
+    const size_t in_bytes = nR + ny + nsq;
+    rc = ensure_stage(*c, in_bytes);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(c->h_stage, R, nR);
+    memcpy(c->h_stage + nR, y, ny);
+    double* sq = (double*)(c->h_stage + nR + ny);
+    for (int k = 0; k < maxcut; ++k) sq[k] = std::sqrt((double)k);
+    void* G = nullptr;
+    if (hipMalloc(&G, (size_t)total * 16) != hipSuccess)
+        return fail(WH_ENOMEM, "device tensor of %lld entries", (long long)total);
+    hipError_t e = hipMemcpyAsync(c->d_stage, c->h_stage, in_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(G, C, 16, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = wh_launch_hermite((int)n, cut, renorm, c->d_stage, c->d_stage + nR, (const double*)(c->d_stage + nR + ny), G,
+                              c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, G, (size_t)total * 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(G);
+    if (e != hipSuccess) return fail(WH_EDEVICE, "hermite recursion: %s", hipGetErrorString(e));
+    return WH_OK;
+}
+
 }  // extern "C"

@@ -19,3 +19,5 @@ hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partial
 size_t wh_lhb_lds_bytes(int MM);
 hipError_t wh_launch_lhb(const wh_lhb_job* jobs, int njobs, const void* pool, void* partials,
                          int64_t nblocks, int MM, void* out, hipStream_t stream);
+hipError_t wh_launch_hermite(int n, const int* cut, int renorm, const void* R, const void* y, const double* sq,
+                             void* G, hipStream_t stream);
