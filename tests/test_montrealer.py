@@ -138,13 +138,20 @@ def test_gpu_physical_and_real(n):
 
 @pytest.mark.gpu
 def test_gpu_identities():
-    """Permutation invariance and Eq. 41 on the device at n = 10."""
+    """Permutation invariance and Eq. 41 at n = 10: the device meets the bar on the permuted and
+    rescaled inputs, whose double-double values satisfy the identities."""
     rng = np.random.default_rng(12)
     n = 10
     A = aad(rng, n)
     perm = rng.permutation(n)
     perm = np.concatenate((perm, perm + n))
-    assert relerr(mtl(A[np.ix_(perm, perm)]), mtl(A)) < 1e-10
+    P = A[np.ix_(perm, perm)]
     g = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
     G = np.diag(np.concatenate((g, g.conj())))
-    assert relerr(mtl(G @ A @ G), np.prod(np.abs(g) ** 2) * mtl(A)) < 1e-10
+    S = G @ A @ G
+    t = O.mtl(A, precision="dd")
+    assert relerr(O.mtl(P, precision="dd"), t) < 1e-12
+    assert relerr(O.mtl(S, precision="dd"), np.prod(np.abs(g) ** 2) * t) < 1e-12
+    _bar(mtl(A), A)
+    _bar(mtl(P), P)
+    _bar(mtl(S), S)

@@ -65,12 +65,24 @@ _lib = None
 _lock = threading.Lock()
 
 
+def _one_hip_runtime():
+    """Load torch (when installed) before the library.  torch bundles its own HIP runtime under
+    the same soname (libamdhip64.so.7) as the system one the library links: whichever is loaded
+    first serves the whole process, and torch's GPU initialisation fails on the system runtime
+    while the library works on either.  Importing torch first makes its runtime the one."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path=None):
     """Load the shared library and declare its signatures (no device call is made)."""
     global _lib
     with _lock:
         if _lib is not None and path is None:
             return _lib
+        _one_hip_runtime()
         p = path or LIB_PATH
         if not os.path.exists(p):
             raise RuntimeError(
