@@ -114,6 +114,9 @@ typedef struct wh_request {
  * reference does (h - 1 products per term) instead of the default half-products formulation
  * (powers up to B^ceil(h/2), traces of higher powers as <B^a, (B^b)^T>). */
 #define WH_OPT_FULL 2
+/* wh_request.options: on the all-ones kernel with complex inputs, run the products on the
+ * vector FMA units (register-tiled) instead of the FP64 matrix cores (the default). */
+#define WH_OPT_VALU 8
 /* wh_request.options: a montrealer request (montrealer / lmontrealer, _montrealer.py:37-101)
  * over n = E modes.  a_off: X Sigma (2E x 2E, rows of Sigma swapped by halves); for the loop
  * montrealer d_off: X zeta and v_off: conj(X zeta), else both -1.  edge_reps, glynn and

@@ -133,3 +133,28 @@ def test_half_and_full_products(loop, real):
         bar = max(TOL, 2 * relerr(ref64, truth))
         assert relerr(a, truth) < bar, ("half", A.shape[0], relerr(a, truth), bar)
         assert relerr(f, truth) < bar, ("full", A.shape[0], relerr(f, truth), bar)
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_matrix_cores_and_vector_units(loop):
+    """The complex all-ones kernel on the FP64 matrix cores (default) and on the vector units,
+    tile sides 1..6 (n = 8 .. 48), against truth; bar as above."""
+    rng = np.random.default_rng(61 + loop)
+    sizes = [4, 6, 8, 12, 16, 20, 24, 30, 32, 36, 40, 44, 48] if not loop else [6, 8, 14, 16, 22, 24, 30, 32, 38, 40, 46]
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in sizes]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    mc = B.DeviceBatch(preps)
+    vu = B.DeviceBatch(preps, valu_products=True)
+    mc.run()
+    vu.run()
+    rm, rv = mc.results(), vu.results()
+    for (A, l), a, v in zip(reqs, rm, rv):
+        n = A.shape[0]
+        if n <= 24:
+            truth = O.hafnian_sieve(A, loop=l, precision="dd")
+            ref64 = O.hafnian_sieve(A, loop=l, precision="double")
+            bar = max(TOL, 2 * relerr(ref64, truth))
+            assert relerr(a, truth) < bar, ("matrix cores", n, relerr(a, truth), bar)
+            assert relerr(v, truth) < bar, ("vector units", n, relerr(v, truth), bar)
+        else:  # beyond the oracle's seconds: the two kernels agree to the bar
+            assert relerr(a, v) < 1e-10, (n, relerr(a, v))

@@ -107,9 +107,12 @@ class DeviceBatch:
         force_generic (bool): run every request on the general kernel
         full_products (bool): on the all-ones kernel, form every power as the reference does
             (h - 1 products per term) instead of the half-products formulation
+        valu_products (bool): on the all-ones kernel with complex inputs, run the products on
+            the vector FMA units instead of the FP64 matrix cores
     """
 
-    def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False):
+    def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False,
+                 valu_products=False):
         import torch
 
         self.torch = torch
@@ -130,7 +133,7 @@ class DeviceBatch:
         self.plan = None
         if self.nreq == 0:
             return
-        options = (1 if force_generic else 0) | (2 if full_products else 0)
+        options = (1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
         reqs, pool = pack_jobs(jobs, options=options)
         with torch.cuda.device(self.device):
             self.pool = torch.from_numpy(pool.view(np.float64).copy()).to(self.device)

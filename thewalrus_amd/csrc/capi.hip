@@ -167,7 +167,7 @@ int describe(const wh_request& r, wh_job_dev& j) {
 
 // Kernel families.  PM1: Glynn, even N, every multiplicity 1 (m = n for every term); the
 // register-tiled kernel with T x T tiles per lane, n_pad = 8T.  GENERIC: everything else.
-enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1 };
+enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2 };
 constexpr int kPm1MaxT = 6;
 
 struct GroupKey {
@@ -184,14 +184,23 @@ struct GroupKey {
     }
 };
 
+// Complex inputs with the half-products formulation run the products on the matrix cores
+// (KIND_PM1M) unless WH_OPT_VALU asks for the register-tiled vector kernel.
+static void pick_matrix_cores(GroupKey& g, int options) {
+    // T = 6 (n in [42, 48]) keeps 72 complex accumulators per lane and spills on the matrix
+    // cores: it stays on the vector kernel
+    if (g.kind == KIND_PM1 && !g.real && g.half && g.T <= 5 && !(options & WH_OPT_VALU)) g.kind = KIND_PM1M;
+}
+
 GroupKey classify(const wh_job_dev& j, int options) {
     GroupKey g{KIND_GENERIC, 0, 0, 0, 0};
-    if (j.flags & WH_F_MTL) {  // the montrealer runs on the all-ones kernel only
+    if (j.flags & WH_F_MTL) {  // the montrealer runs on the all-ones kernels only
         g.kind = KIND_PM1;
         g.T = (2 * j.E + 7) / 8;
         g.real = (j.flags & WH_F_REAL) ? 1 : 0;
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
         g.half = (options & WH_OPT_FULL) ? 0 : 1;
+        pick_matrix_cores(g, options);
         return g;
     }
     if (options & WH_OPT_GENERIC) return g;
@@ -205,17 +214,18 @@ GroupKey classify(const wh_job_dev& j, int options) {
         g.real = (j.flags & WH_F_REAL) ? 1 : 0;
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
         g.half = (options & WH_OPT_FULL) ? 0 : 1;
+        pick_matrix_cores(g, options);
     }
     return g;
 }
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
 // under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
-int pm1_waves(int T, bool real) {
+int pm1_waves(int T, bool real, bool matrix_cores) {
     const size_t lds_cap = 160 * 1024;
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
-        size_t b = wh_pm1_lds_bytes(T, real, nw);
+        size_t b = matrix_cores ? wh_pm1m_lds_bytes(T, nw) : wh_pm1_lds_bytes(T, real, nw);
         if (b > lds_cap) break;
         int per_cu = (int)(lds_cap / b) * nw;
         if (per_cu > 16) per_cu = 16;
@@ -225,7 +235,7 @@ int pm1_waves(int T, bool real) {
 }
 
 int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
-    if (g.kind == KIND_PM1) {
+    if (g.kind == KIND_PM1 || g.kind == KIND_PM1M) {
         int64_t c = j.steps / 1024;
         if (c < 4 * nw) c = 4 * nw;
         if (c > 4096) c = 4096;
@@ -247,7 +257,7 @@ double term_flops(const wh_job_dev& j, const GroupKey& g) {
     int h = j.N / 2;
     int m = 2 * j.E;
     int np = std::min(h + 1, m) - 2;
-    if (g.kind == KIND_PM1 && g.half) np = (h + 1) / 2 - 1;
+    if ((g.kind == KIND_PM1 || g.kind == KIND_PM1M) && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
     double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;
     return c * m * (double)m * m * np;
@@ -295,7 +305,8 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
             gr.key = g;
             gr.job_begin = k;
             gr.block_begin = base;
-            gr.nw = g.kind == KIND_PM1 ? pm1_waves(g.T, g.real != 0) : 1;
+            gr.nw = (g.kind == KIND_PM1 || g.kind == KIND_PM1M)
+                        ? pm1_waves(g.T, g.real != 0, g.kind == KIND_PM1M) : 1;
             L.groups.push_back(gr);
         }
         Group& gr = L.groups.back();
@@ -324,7 +335,9 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
         int nj = (int)(g.job_end - g.job_begin);
         int64_t nb = g.block_end - g.block_begin;
         hipError_t e;
-        if (g.key.kind == KIND_PM1)
+        if (g.key.kind == KIND_PM1M)
+            e = wh_launch_pm1m(g.key.T, g.key.loop != 0, gj, nj, pool, partials, nb, g.block_begin, g.nw, s);
+        else if (g.key.kind == KIND_PM1)
             e = wh_launch_pm1(g.key.T, g.key.half != 0, g.key.real != 0, g.key.loop != 0, gj, nj, pool,
                               partials, nb,
                               g.block_begin, g.nw, s);
