@@ -20,8 +20,10 @@
 
 namespace {
 
-#ifndef WH_PM1_PAD
-#define WH_PM1_PAD 1
+// LDS row stride 8T + 2 (complex elements): the 16-lane groups of the accumulator stores and
+// of the operand reads then hit distinct banks (the transposed trace reads stay 2-way)
+#ifndef WH_PM1M_PAD
+#define WH_PM1M_PAD 2
 #endif
 
 static __constant__ double kInvJm[WH_EMAX + 1] = {
@@ -84,9 +86,36 @@ __device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
     return group8_sum(s3);
 }
 
+// [eta^h] exp(sum_i a_i eta^i) for h <= H by e_0 = 1, e_k = (1/k) sum_(i=1..k) b_i e_(k-i) with
+// b_i = i a_i = p_i / 2 (+ i v_i / 2 with loops), the exponential's recurrence; two partial
+// sums per k halve the dependent chain.
+template <int H, bool LOOP>
+__device__ __forceinline__ cplx coeff_top(const cplx* pt, const cplx* lv, int h) {
+    cplx b[H + 1], e[H + 1];
+#pragma unroll
+    for (int i = 1; i <= H; ++i) {
+        cplx v = cmulr(pt[i], 0.5);
+        if (LOOP) v = cadd(v, cmulr(lv[i], 0.5 * i));
+        b[i] = i <= h ? v : cmk(0.0, 0.0);
+    }
+    e[0] = cmk(1.0, 0.0);
+    cplx res = e[0];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {  // every k to H (fixed trip count keeps e[] in registers)
+        cplx s0 = cmk(0.0, 0.0), s1 = cmk(0.0, 0.0);
+#pragma unroll
+        for (int i = 1; i <= k; i += 2) cmac(s0, b[i], e[k - i]);
+#pragma unroll
+        for (int i = 2; i <= k; i += 2) cmac(s1, b[i], e[k - i]);
+        e[k] = cmulr(cadd(s0, s1), kInvJm[k]);
+        if (k == h) res = e[k];
+    }
+    return res;
+}
+
 template <int T> struct Geo {
     static constexpr int NP = 8 * T;
-    static constexpr int S = NP + WH_PM1_PAD;  // LDS row stride (complex elements)
+    static constexpr int S = NP + WH_PM1M_PAD;  // LDS row stride (complex elements)
 };
 
 struct LaneMap {
@@ -340,26 +369,42 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
     for (int q = 0; q < T; ++q) E.ys[q] = cmk(0.0, 0.0);
     const cplx* pa = Pw + (4 * L.br + L.x) * S + L.y;
     const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
+    // operands of chunk kc + 1 are loaded while chunk kc runs (register double buffer), the
+    // epilogue operands of row r one chunk before the row's first step
+    cplx avn[T], bqn[T];
+#pragma unroll
+    for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S];
+#pragma unroll
+    for (int q = 0; q < T; ++q) bqn[q] = pb[8 * q];
+#pragma unroll
+    for (int q = 0; q < T; ++q) E.tx[q] = Pw[L.cd(q) * S + L.rd(0)];
+    if (LOOP) {
+#pragma unroll
+        for (int q = 0; q < T; ++q) E.dv[q] = Dv[L.cd(q)];
+        E.xd = cmulr(XD0[L.rd(0)], col_factor(tu, L.rd(0), h, unset));
+    }
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
         cplx av[T], bq[T];
         double nbi[T];
 #pragma unroll
-        for (int p = 0; p < T; ++p) av[p] = pa[8 * p * S + 4 * kc];
+        for (int p = 0; p < T; ++p) av[p] = avn[p];
 #pragma unroll
         for (int q = 0; q < T; ++q) {
-            bq[q] = cmulr(pb[4 * kc * S + 8 * q], sg[q]);
+            bq[q] = cmulr(bqn[q], sg[q]);
             nbi[q] = -bq[q].im;
         }
-        if ((kc & 1) == 0) {  // the row's epilogue operands
-            const int r = kc >> 1;
+        if (kc + 1 < NK) {
+#pragma unroll
+            for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S + 4 * (kc + 1)];
+#pragma unroll
+            for (int q = 0; q < T; ++q) bqn[q] = pb[4 * (kc + 1) * S + 8 * q];
+        }
+        if ((kc & 1) == 1 && kc + 1 < NK) {  // next row's epilogue operands
+            const int r = (kc + 1) >> 1;
 #pragma unroll
             for (int q = 0; q < T; ++q) E.tx[q] = Pw[L.cd(q) * S + L.rd(r)];
-            if (LOOP) {
-#pragma unroll
-                for (int q = 0; q < T; ++q) E.dv[q] = Dv[L.cd(q)];
-                E.xd = cmulr(XD0[L.rd(r)], col_factor(tu, L.rd(r), h, unset));
-            }
+            if (LOOP) E.xd = cmulr(XD0[L.rd(r)], col_factor(tu, L.rd(r), h, unset));
         }
         const int nst = epi_steps<T, LOOP>(kc);
         int st = 0;
@@ -623,27 +668,12 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
             }
             continue;
         }
-        // coefficient of eta^h in exp(sum_i a_i eta^i) (_hafnian.py:200-208, :232-241)
-        cplx* oldc = comb0;
-        cplx* newc = comb1;
-        if (lane <= h) oldc[lane] = lane == 0 ? cmk(1.0, 0.0) : cmk(0.0, 0.0);
-        for (int i = 1; i <= h; ++i) {
-            cplx f = cdivr(pt[i], (double)(2 * i));
-            if (LOOP) f = cadd(f, cdivr(lv[i], 2.0));
-            if (lane <= h) {
-                cplx val = oldc[lane];
-                cplx pf = cmk(1.0, 0.0);
-                for (int j = 1; j <= h / i; ++j) {
-                    pf = cmul(pf, cmulr(f, kInvJm[j]));
-                    if (i * j <= lane) val = cadd(val, cmul(oldc[lane - i * j], pf));
-                }
-                newc[lane] = val;
-            }
-            cplx* tmp = oldc;
-            oldc = newc;
-            newc = tmp;
-        }
-        if (lane == h) acc_sum.add(cmulr(oldc[h], prefac));
+        // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2): the
+        // reference builds it factor by factor (_hafnian.py:200-208, :232-241); here every
+        // lane runs the power-series exponential e_k = (1/k) sum_(i<=k) (i a_i) e_(k-i) in
+        // registers (the same polynomial, no LDS traffic on the wave's serial tail)
+        const cplx top = coeff_top<4 * T, LOOP>(pt, lv, h);
+        if (lane == h) acc_sum.add(cmulr(top, prefac));
     }
 
     // fold the waves of this workgroup in wave order
@@ -679,7 +709,7 @@ wh_sieve_pm1m(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
 // LDS of the matrix-core kernel: M and one power per wave (as the vector kernel), the
 // vectors XD0 and D, and per wave pt, lv, two coefficient rows and two z rows.
 size_t wh_pm1m_lds_bytes(int T, int nw) {
-    size_t NP = 8 * (size_t)T, S = NP + WH_PM1_PAD;
+    size_t NP = 8 * (size_t)T, S = NP + WH_PM1M_PAD;
     size_t mat = (NP * S * sizeof(cplx) + 31) / 32 * 32;
     size_t small = (4 * (WH_EMAX + 2) + 2 * NP) * sizeof(cplx);
     return (1 + nw) * mat + 2 * NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
