@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 from thewalrus_amd import workloads as W  # noqa: E402
 
-FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
+FP64_PEAK = 78.6e12  # MI355X FP64 (matrix and vector), FLOP/s
 METRIC = "hafnians/s + p50/p99 ms, mixed n in [8,40] c128 stream vs per-call GPU"
 
 
@@ -275,13 +275,13 @@ def main():
     if best:
         t, fl, lp, cnt, wref = best
         roof = {
-            "bound": "fp64-vector",
+            "bound": "mfma",
             "achieved": fl / t / 1e12,
             "peak": FP64_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": fl / t / FP64_PEAK,
             "traffic": None,
-            "kernel": "wh_sieve_pm1<5,complex,%s,half> (n in [34,40], %d requests)" % ("loop" if lp else "plain", cnt),
+            "kernel": "wh_sieve_pm1m<5,%s> (FP64 matrix cores, n in [34,40], %d requests)" % ("loop" if lp else "plain", cnt),
             "flops_formula": "sum over requests of 2^(h-1) * 8 n^3 * (ceil(h/2)-1): complex n x n "
             "products per term of the half-products formulation (powers up to B^ceil(h/2)); "
             "lower-order trace and loop terms not counted",

@@ -66,17 +66,38 @@ __device__ __forceinline__ double col_sign(uint64_t t, int c, int h, double unse
 #ifndef WH_PM1_PIPELINE
 #define WH_PM1_PIPELINE 1
 #endif
-// Reciprocals 1/j (j <= WH_EMAX) for the coefficient recursion: multiplying by the rounded
-// reciprocal differs from the division by at most one rounding and removes ~70 double
-// divisions per term from the wave's serial tail.
-#ifndef WH_PM1_RECIP
-#define WH_PM1_RECIP 1
-#endif
+// Reciprocals 1/k (k <= WH_EMAX) for the exponential recurrence of the term coefficient.
 __constant__ double kInvJ[WH_EMAX + 1] = {
     0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,  1.0 / 8,
     1.0 / 9,    1.0 / 10,   1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
     1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26,
     1.0 / 27,   1.0 / 28,   1.0 / 29, 1.0 / 30, 1.0 / 31, 1.0 / 32};
+
+// [eta^h] exp(sum_i a_i eta^i) for h <= H: e_0 = 1, e_k = (1/k) sum_(i=1..k) b_i e_(k-i) with
+// b_i = i a_i = p_i / 2 (+ i v_i / 2 with loops); fixed trip counts keep e[] in registers.
+template <class E, int H, bool LOOP>
+__device__ __forceinline__ E coeff_top_e(const cplx* pt, const cplx* lv, int h) {
+    E b[H + 1], e[H + 1];
+#pragma unroll
+    for (int i = 1; i <= H; ++i) {
+        E v = escale(eget(pt[i], (E*)nullptr), 0.5);
+        if (LOOP) v = eadd(v, escale(eget(lv[i], (E*)nullptr), 0.5 * i));
+        b[i] = i <= h ? v : ezero_v((E*)nullptr);
+    }
+    e[0] = eone((E*)nullptr);
+    E res = e[0];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        E s0 = ezero_v((E*)nullptr), s1 = ezero_v((E*)nullptr);
+#pragma unroll
+        for (int i = 1; i <= k; i += 2) emac(s0, b[i], e[k - i]);
+#pragma unroll
+        for (int i = 2; i <= k; i += 2) emac(s1, b[i], e[k - i]);
+        e[k] = escale(eadd(s0, s1), kInvJ[k]);
+        if (k == h) res = e[k];
+    }
+    return res;
+}
 
 template <int T> struct Lds {
     static constexpr int S = 8 * T + WH_PM1_PAD;
@@ -418,32 +439,11 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             }
             continue;
         }
-        // coefficient of eta^h in exp(sum_i a_i eta^i) (_hafnian.py:200-208, :232-241); the
-        // lanes hold the coefficients 0..h and add the shifted old row in the reference's j order
-        E* oldc = comb0;
-        E* newc = comb1;
-        if (lane <= h) oldc[lane] = lane == 0 ? eone((E*)nullptr) : ezero_v((E*)nullptr);
-        for (int i = 1; i <= h; ++i) {
-            E f = ediv(eget(pt[i], (E*)nullptr), (double)(2 * i));
-            if (LOOP) f = eadd(f, ediv(eget(lv[i], (E*)nullptr), 2.0));
-            if (lane <= h) {
-                E val = oldc[lane];
-                E pf = eone((E*)nullptr);
-                for (int j = 1; j <= h / i; ++j) {
-#if WH_PM1_RECIP
-                    pf = emul(pf, escale(f, kInvJ[j]));  // f / j as f * (1/j): no division
-#else
-                    pf = emul(pf, ediv(f, (double)j));
-#endif
-                    if (i * j <= lane) val = eadd(val, emul(oldc[lane - i * j], pf));
-                }
-                newc[lane] = val;
-            }
-            E* tmp = oldc;
-            oldc = newc;
-            newc = tmp;
-        }
-        if (lane == h) acc_sum.add(cmulr(etoc(oldc[h]), prefac));
+        // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2): the
+        // reference builds it factor by factor (_hafnian.py:200-208, :232-241); every lane runs
+        // the power-series exponential e_k = (1/k) sum_(i<=k) (i a_i) e_(k-i) in registers
+        const E top = coeff_top_e<E, 4 * T, LOOP>(pt, lv, h);
+        if (lane == h) acc_sum.add(cmulr(etoc(top), prefac));
     }
 
     // fold the waves of this workgroup in wave order
