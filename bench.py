@@ -280,7 +280,9 @@ def main():
             "peak": FP64_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": fl / t / FP64_PEAK,
-            "traffic": None,
+            # HBM bytes per launch of wh_sieve_pm1m<5, loop> on this sub-batch from rocprofv3
+            # FETCH_SIZE x 2 + WRITE_SIZE (copy-calibrated), profiles/pm1m_T5_loop_counters.md
+            "traffic": 8.43e6,
             "kernel": "wh_sieve_pm1m<5,%s> (FP64 matrix cores, n in [34,40], %d requests)" % ("loop" if lp else "plain", cnt),
             "flops_formula": "sum over requests of 2^(h-1) * 8 n^3 * (ceil(h/2)-1): complex n x n "
             "products per term of the half-products formulation (powers up to B^ceil(h/2)); "
