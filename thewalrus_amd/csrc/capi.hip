@@ -171,16 +171,17 @@ enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2 };
 constexpr int kPm1MaxT = 6;
 
 struct GroupKey {
-    int kind, T, real, loop, half;
+    int kind, T, real, loop, half, mtl;
     bool operator<(const GroupKey& o) const {
         if (kind != o.kind) return kind < o.kind;
         if (T != o.T) return T < o.T;
         if (real != o.real) return real < o.real;
         if (loop != o.loop) return loop < o.loop;
-        return half < o.half;
+        if (half != o.half) return half < o.half;
+        return mtl < o.mtl;
     }
     bool operator==(const GroupKey& o) const {
-        return kind == o.kind && T == o.T && real == o.real && loop == o.loop && half == o.half;
+        return kind == o.kind && T == o.T && real == o.real && loop == o.loop && half == o.half && mtl == o.mtl;
     }
 };
 
@@ -193,9 +194,10 @@ static void pick_matrix_cores(GroupKey& g, int options) {
 }
 
 GroupKey classify(const wh_job_dev& j, int options) {
-    GroupKey g{KIND_GENERIC, 0, 0, 0, 0};
+    GroupKey g{KIND_GENERIC, 0, 0, 0, 0, 0};
     if (j.flags & WH_F_MTL) {  // the montrealer runs on the all-ones kernels only
         g.kind = KIND_PM1;
+        g.mtl = 1;
         g.T = (2 * j.E + 7) / 8;
         g.real = (j.flags & WH_F_REAL) ? 1 : 0;
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
@@ -336,7 +338,8 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
         int64_t nb = g.block_end - g.block_begin;
         hipError_t e;
         if (g.key.kind == KIND_PM1M)
-            e = wh_launch_pm1m(g.key.T, g.key.loop != 0, gj, nj, pool, partials, nb, g.block_begin, g.nw, s);
+            e = wh_launch_pm1m(g.key.T, g.key.loop != 0, g.key.mtl != 0, gj, nj, pool, partials, nb, g.block_begin,
+                               g.nw, s);
         else if (g.key.kind == KIND_PM1)
             e = wh_launch_pm1(g.key.T, g.key.half != 0, g.key.real != 0, g.key.loop != 0, gj, nj, pool,
                               partials, nb,

@@ -193,73 +193,20 @@ __device__ __forceinline__ void mfma_store(cplx* X, const double (&aR)[T][T], co
         for (int q = 0; q < T; ++q) X[L.rd(p) * S + L.cd(q)] = cmk(aR[p][q], aI[p][q]);
 }
 
-#ifndef WH_PM1M_PIPE
-#define WH_PM1M_PIPE 1
-#endif
-#ifndef WH_PM1M_SCHED
-#define WH_PM1M_SCHED 0
-#endif
 
-// Per-power epilogue state: the traces and loop scalars of power a, accumulated row group by
-// row group so that the work can run between the matrix-core instructions of the next product.
-struct EpiState {
-    cplx qe, de, dodd;
-};
-
-// Row group p of power a's epilogue: <B^a, (B^a)^T> over the row, z_a for the row (written
-// to zcur), and the row's share of y_a = XD_d B^a (ys).
-template <int T, bool LOOP>
-__device__ __forceinline__ void epi_row(int p, const double (&aR)[T][T], const double (&aI)[T][T],
-                                        const LaneMap& L, const cplx* Pw, const cplx* Dv, const cplx* XD0,
-                                        cplx* zcur, uint64_t tu, int h, double unset, EpiState& E,
-                                        cplx (&ys)[T]) {
-    constexpr int S = Geo<T>::S;
-#pragma unroll
-    for (int q = 0; q < T; ++q) cmac(E.qe, cmk(aR[p][q], aI[p][q]), Pw[L.cd(q) * S + L.rd(p)]);
-    if (LOOP) {
-        cplx zs = cmk(0.0, 0.0);
-#pragma unroll
-        for (int q = 0; q < T; ++q) cmac(zs, cmk(aR[p][q], aI[p][q]), Dv[L.cd(q)]);
-        zs = cmk(sum_x_bc(zs.re), sum_x_bc(zs.im));
-        zcur[L.rd(p)] = zs;  // the 8 lanes of the row hold the same sum: same value, same address
-        const cplx xd = cmulr(XD0[L.rd(p)], col_factor(tu, L.rd(p), h, unset));
-#pragma unroll
-        for (int q = 0; q < T; ++q) cmac(ys[q], xd, cmk(aR[p][q], aI[p][q]));
-    }
-}
-
-// End of power a's epilogue: the dots of y_a with z_(a-1) and z_a, one eight-value wave sum,
-// and the results into pt / lv.
-template <int T, bool LOOP>
-__device__ __forceinline__ void epi_final(int a, cplx qo, const LaneMap& L, int lane, const cplx* zprev,
-                                          const cplx* zcur, EpiState& E, const cplx (&ys)[T], cplx* pt,
-                                          cplx* lv) {
-    if (LOOP) {
-#pragma unroll
-        for (int q = 0; q < T; ++q) {
-            cmac(E.de, ys[q], zprev[L.cd(q)]);
-            cmac(E.dodd, ys[q], zcur[L.cd(q)]);
-        }
-    }
-    const double vals[8] = {qo.re, qo.im, E.qe.re, E.qe.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
-    const double tot = wave_sum8(vals, lane);
-    // lane 8k holds value k: p_ko, p_ke, v_2a, v_(2a+1) (re, im); the other lanes write the
-    // spare slot pt[WH_EMAX + 1] (never read), so the store needs no branch
-    const int k = lane >> 3;
-    const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
-    double* dst = k < 2 ? &pt[ko].re : k < 4 ? &pt[ke].re : k < 6 ? &lv[2 * a].re : &lv[2 * a + 1].re;
-    dst += k & 1;
-    if ((lane & 7) != 0) dst = &pt[WH_EMAX + 1].re;
-    *dst = tot;
-}
-
-// Epilogue of power a_prev cut into micro-steps that run between the matrix-core instructions
-// of the next product: after every pair of MFMAs the wave issues one micro-step, and a
-// scheduling barrier keeps the compiler from gathering the MFMAs back into one block (the wave
-// issues in order, so only instructions placed between MFMAs run in their shadow).
-// k chunk kc works on row group r = kc / 2: the even chunk takes the row's trace, z and y
-// products, the odd chunk reduces and stores the row's z; the last chunk then dots y with z
-// and sums the eight results over the wave.
+// Epilogue of power a (from its accumulators and Pw = B^a), cut into micro-steps that run
+// between the matrix-core instructions of the next product: after every pair of MFMAs the
+// wave issues one micro-step, and a scheduling barrier keeps the compiler from gathering the
+// MFMAs back into one block (the wave issues in order, so only instructions placed between
+// MFMAs run in their shadow).  Chunk kc works on row group r = kc / 2: the even chunk takes
+// the row's share of <B^a, (B^a)^T> and of z_a = B^a D, the odd chunk reduces and stores the
+// row's z; the last chunk forms the loop dots and sums the eight results over the wave.
+//
+// Loop dots.  With A symmetric, M = A X and X D_d = D_d X (d repeats over the two halves),
+// (B^a)^T X D_d = X D_d B^a, so y_a = XD_d B^a = X D_d z_a when the loop weights are
+// XD0 = X D (hafnians): v_2a = y_a . z_(a-1) = sum_j d_j z_a[swap j] z_(a-1)[j] and v_(2a+1)
+// likewise with z_a, from the z vectors alone.  The montrealer's weights (conj(X zeta),
+// X zeta) are not of that form: YS keeps each lane's partial y_a over its rows instead.
 template <int T>
 struct EpiPipe {
     cplx qe, de, dodd, zs, xd;
@@ -267,20 +214,42 @@ struct EpiPipe {
     double s1[4], s2[2], s3;
 };
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool YS>
 __device__ __forceinline__ int epi_steps(int kc) {
     constexpr int NK = 2 * T;
     int n = 0;
-    if ((kc & 1) == 0) n = LOOP ? 3 * T : T;
+    if ((kc & 1) == 0) n = LOOP ? (YS ? 3 * T : 2 * T) : T;
     else n = LOOP ? 4 : 0;
-    if (kc == NK - 1) n += (LOOP ? 1 + T : 0) + 4 + 2 + 1 + 3 + 1;
+    if (kc == NK - 1) n += (LOOP ? (YS ? 1 + T : 2) : 0) + 11;
     return n;
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool YS>
+__device__ __forceinline__ void epi_init(EpiPipe<T>& E, const LaneMap& L, const cplx* Dv) {
+    E.qe = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < T; ++q) E.ys[q] = cmk(0.0, 0.0);
+    if (LOOP) {
+#pragma unroll
+        for (int q = 0; q < T; ++q) E.dv[q] = Dv[L.cd(q)];
+    }
+}
+
+// the operands of row group r's micro-steps
+template <int T, bool LOOP, bool YS>
+__device__ __forceinline__ void epi_load_row(int r, EpiPipe<T>& E, const LaneMap& L, const cplx* Pw,
+                                             const cplx* XD0, uint64_t tu, int h, double unset) {
+    constexpr int S = Geo<T>::S;
+#pragma unroll
+    for (int q = 0; q < T; ++q) E.tx[q] = Pw[L.cd(q) * S + L.rd(r)];
+    if (LOOP && YS) E.xd = cmulr(XD0[L.rd(r)], col_factor(tu, L.rd(r), h, unset));
+}
+
+template <int T, bool LOOP, bool YS>
 __device__ __forceinline__ void epi_step(int kc, int st, EpiPipe<T>& E, const double (&oR)[T][T],
                                          const double (&oI)[T][T], const LaneMap& L, int lane, int a_prev,
-                                         cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt, cplx* lv) {
+                                         cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt, cplx* lv,
+                                         uint64_t tu, int h, double unset) {
     constexpr int NK = 2 * T;
     const int r = kc >> 1;
     if ((kc & 1) == 0) {
@@ -292,7 +261,7 @@ __device__ __forceinline__ void epi_step(int kc, int st, EpiPipe<T>& E, const do
             cmac(E.zs, cmk(oR[r][st - T], oI[r][st - T]), E.dv[st - T]);
             return;
         }
-        if (LOOP && st < 3 * T) {
+        if (LOOP && YS && st < 3 * T) {
             cmac(E.ys[st - 2 * T], E.xd, cmk(oR[r][st - 2 * T], oI[r][st - 2 * T]));
             return;
         }
@@ -307,18 +276,37 @@ __device__ __forceinline__ void epi_step(int kc, int st, EpiPipe<T>& E, const do
     }
     if (kc != NK - 1) return;
     if (LOOP) {
-        if (st == 0) {
+        if (YS) {
+            if (st == 0) {
 #pragma unroll
-            for (int q = 0; q < T; ++q) E.zc[q] = zcur[L.cd(q)];
-            return;
+                for (int q = 0; q < T; ++q) E.zc[q] = zcur[L.cd(q)];
+                return;
+            }
+            if (st <= T) {
+                const int q = st - 1;
+                cmac(E.de, E.ys[q], zprev[L.cd(q)]);
+                cmac(E.dodd, E.ys[q], E.zc[q]);
+                return;
+            }
+            st -= 1 + T;
+        } else {
+            // lane e: d_e z_a[swap e] times z_(a-1)[e] and z_a[e]
+            const int n = 2 * h;
+            const int e = lane < n ? lane : 0;
+            if (st == 0) {
+                E.zc[0] = zcur[e];
+                E.zc[1] = zprev[e];
+                E.zc[2] = zcur[e < h ? e + h : e - h];
+                return;
+            }
+            if (st == 1) {
+                const cplx f = cmulr(E.zc[2], lane < n ? col_factor(tu, e, h, unset) : 0.0);
+                E.de = cmul(f, E.zc[1]);
+                E.dodd = cmul(f, E.zc[0]);
+                return;
+            }
+            st -= 2;
         }
-        if (st <= T) {
-            const int q = st - 1;
-            cmac(E.de, E.ys[q], zprev[L.cd(q)]);
-            cmac(E.dodd, E.ys[q], E.zc[q]);
-            return;
-        }
-        st -= 1 + T;
     }
     const double v[8] = {qo_prev.re, qo_prev.im, E.qe.re, E.qe.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
     if (st < 4) { E.s1[st] = dswap32(v[st], v[st + 4]); return; }
@@ -346,8 +334,9 @@ __device__ __forceinline__ void epi_step(int kc, int st, EpiPipe<T>& E, const do
 }
 
 // accN = Pw Ms diag(d) (fully unrolled k chunks) with power a_prev's epilogue (from accO and
-// Pw = B^(a_prev)) in the matrix core's shadow.
-template <int T, bool LOOP>
+// Pw = B^(a_prev)) in the matrix core's shadow.  The operands of chunk kc + 1 load while chunk
+// kc runs (register double buffer), a row's epilogue operands one chunk before its steps.
+template <int T, bool LOOP, bool YS>
 __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, const cplx* __restrict__ Ms,
                                                  double (&nR)[T][T], double (&nI)[T][T], const LaneMap& L,
                                                  const double (&sg)[T], const double (&oR)[T][T],
@@ -364,25 +353,15 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
             nI[p][q] = 0.0;
         }
     EpiPipe<T> E;
-    E.qe = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
-#pragma unroll
-    for (int q = 0; q < T; ++q) E.ys[q] = cmk(0.0, 0.0);
+    epi_init<T, LOOP, YS>(E, L, Dv);
     const cplx* pa = Pw + (4 * L.br + L.x) * S + L.y;
     const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
-    // operands of chunk kc + 1 are loaded while chunk kc runs (register double buffer), the
-    // epilogue operands of row r one chunk before the row's first step
     cplx avn[T], bqn[T];
 #pragma unroll
     for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S];
 #pragma unroll
     for (int q = 0; q < T; ++q) bqn[q] = pb[8 * q];
-#pragma unroll
-    for (int q = 0; q < T; ++q) E.tx[q] = Pw[L.cd(q) * S + L.rd(0)];
-    if (LOOP) {
-#pragma unroll
-        for (int q = 0; q < T; ++q) E.dv[q] = Dv[L.cd(q)];
-        E.xd = cmulr(XD0[L.rd(0)], col_factor(tu, L.rd(0), h, unset));
-    }
+    epi_load_row<T, LOOP, YS>(0, E, L, Pw, XD0, tu, h, unset);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
         cplx av[T], bq[T];
@@ -400,13 +379,8 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
 #pragma unroll
             for (int q = 0; q < T; ++q) bqn[q] = pb[4 * (kc + 1) * S + 8 * q];
         }
-        if ((kc & 1) == 1 && kc + 1 < NK) {  // next row's epilogue operands
-            const int r = (kc + 1) >> 1;
-#pragma unroll
-            for (int q = 0; q < T; ++q) E.tx[q] = Pw[L.cd(q) * S + L.rd(r)];
-            if (LOOP) E.xd = cmulr(XD0[L.rd(r)], col_factor(tu, L.rd(r), h, unset));
-        }
-        const int nst = epi_steps<T, LOOP>(kc);
+        if ((kc & 1) == 1 && kc + 1 < NK) epi_load_row<T, LOOP, YS>((kc + 1) >> 1, E, L, Pw, XD0, tu, h, unset);
+        const int nst = epi_steps<T, LOOP, YS>(kc);
         int st = 0;
 #pragma unroll
         for (int p = 0; p < T; ++p)
@@ -414,7 +388,9 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
             for (int q = 0; q < T; ++q) {
                 nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
                 nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
-                if (st < nst) epi_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv);
+                if (st < nst)
+                    epi_step<T, LOOP, YS>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h,
+                                          unset);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -424,33 +400,37 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
             for (int q = 0; q < T; ++q) {
                 nR[p][q] = mfma4(av[p].im, nbi[q], nR[p][q]);
                 nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
-                if (st < nst) epi_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv);
+                if (st < nst)
+                    epi_step<T, LOOP, YS>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h,
+                                          unset);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
-        for (; st < nst; ++st) epi_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv);
+        for (; st < nst; ++st)
+            epi_step<T, LOOP, YS>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h, unset);
     }
 }
 
-// Power a's epilogue on its own (the last power of a term, or a term with a single power).
-template <int T, bool LOOP>
+// Power a's epilogue on its own (the last power of a term): the same micro-steps in sequence.
+template <int T, bool LOOP, bool YS>
 __device__ __forceinline__ void epi_alone(const double (&aR)[T][T], const double (&aI)[T][T], int a, cplx qo,
                                           const LaneMap& L, int lane, const cplx* Pw, const cplx* Dv,
                                           const cplx* XD0, const cplx* zprev, cplx* zcur, uint64_t tu, int h,
                                           double unset, cplx* pt, cplx* lv) {
-    EpiState E;
-    E.qe = cmk(0.0, 0.0);
-    E.de = cmk(0.0, 0.0);
-    E.dodd = cmk(0.0, 0.0);
-    cplx ys[T];
+    constexpr int NK = 2 * T;
+    EpiPipe<T> E;
+    epi_init<T, LOOP, YS>(E, L, Dv);
 #pragma unroll
-    for (int q = 0; q < T; ++q) ys[q] = cmk(0.0, 0.0);
+    for (int kc = 0; kc < NK; ++kc) {
+        if ((kc & 1) == 0) epi_load_row<T, LOOP, YS>(kc >> 1, E, L, Pw, XD0, tu, h, unset);
+        const int nst = epi_steps<T, LOOP, YS>(kc);
 #pragma unroll
-    for (int p = 0; p < T; ++p) epi_row<T, LOOP>(p, aR, aI, L, Pw, Dv, XD0, zcur, tu, h, unset, E, ys);
-    epi_final<T, LOOP>(a, qo, L, lane, zprev, zcur, E, ys, pt, lv);
+        for (int st = 0; st < 3 * T + 2 * T + 16; ++st)
+            if (st < nst) epi_step<T, LOOP, YS>(kc, st, E, aR, aI, L, lane, a, qo, zprev, zcur, pt, lv, tu, h, unset);
+    }
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool YS>
 __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __restrict__ pool,
                                           double4* __restrict__ partials, int64_t b, unsigned char* smem) {
     constexpr int NP = Geo<T>::NP;
@@ -544,7 +524,6 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
             for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];
         }
         const int Ahalf = (h + 1) / 2;
-#if WH_PM1M_PIPE
         // powers up to B^A, A = ceil(h/2): tr(B^(2a)) = <B^a, (B^a)^T>, tr(B^(2a+1)) =
         // <B^(a+1), (B^a)^T>; loop scalars v_(2a) = y_a . z_(a-1), v_(2a+1) = y_a . z_a with
         // y_a = XD_d B^a and z_a = B^a D.  Software pipeline: the product forming B^a runs with
@@ -564,12 +543,12 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
 #pragma unroll
                 for (int q = 0; q < T; ++q) sgp[q] = col_factor(tu, L.cd(q), h, unset);
                 // B^a into b, epilogue of B^(a-1) from a
-                mfma_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
+                mfma_product_epi<T, LOOP, YS>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
                                           Zb + (a & 1) * NP, Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, bR, bI, L);
                 mfma_store<T>(Pw, bR, bI, L);
                 // B^(a+1) into a, epilogue of B^a from b
-                mfma_product_epi<T, LOOP>(Pw, Ms, aR, aI, L, sgp, bR, bI, a, qo, lane, Dv, XD0,
+                mfma_product_epi<T, LOOP, YS>(Pw, Ms, aR, aI, L, sgp, bR, bI, a, qo, lane, Dv, XD0,
                                           Zb + ((a + 1) & 1) * NP, Zb + (a & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, aR, aI, L);
                 mfma_store<T>(Pw, aR, aI, L);
@@ -578,79 +557,17 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
                 double sgp[T];
 #pragma unroll
                 for (int q = 0; q < T; ++q) sgp[q] = col_factor(tu, L.cd(q), h, unset);
-                mfma_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
+                mfma_product_epi<T, LOOP, YS>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
                                           Zb + (a & 1) * NP, Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, bR, bI, L);
                 mfma_store<T>(Pw, bR, bI, L);
-                epi_alone<T, LOOP>(bR, bI, a, qo, L, lane, Pw, Dv, XD0, Zb + ((a - 1) & 1) * NP,
+                epi_alone<T, LOOP, YS>(bR, bI, a, qo, L, lane, Pw, Dv, XD0, Zb + ((a - 1) & 1) * NP,
                                    Zb + (a & 1) * NP, tu, h, unset, pt, lv);
             } else {  // the last power (a - 1) is in a
-                epi_alone<T, LOOP>(aR, aI, a - 1, qo, L, lane, Pw, Dv, XD0, Zb + (a & 1) * NP,
+                epi_alone<T, LOOP, YS>(aR, aI, a - 1, qo, L, lane, Pw, Dv, XD0, Zb + (a & 1) * NP,
                                    Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
             }
         }
-#else
-        // powers up to B^A, A = ceil(h/2): tr(B^(2a)) = <B^a, (B^a)^T>, tr(B^(2a+1)) =
-        // <B^(a+1), (B^a)^T>; loop scalars v_(2a) = y_a . z_(a-1), v_(2a+1) = y_a . z_a with
-        // y_a = XD_d B^a and z_a = B^a D: the lanes' partial products of y_a (over their rows)
-        // with z at their columns sum, over the wave, to the dots.
-        mfma_store<T>(Pw, aR, aI, L);
-        for (int a = 1; a <= Ahalf; ++a) {
-            cplx qo, qe;
-            if (a == 1) {
-                qo = cmk(0.0, 0.0);
-                if (L.diag()) {
-#pragma unroll
-                    for (int p = 0; p < T; ++p) qo = cadd(qo, cmk(aR[p][p], aI[p][p]));
-                }
-                qe = mfma_tdot<T>(Pw, aR, aI, L);
-            } else {
-                double sgp[T];
-#pragma unroll
-                for (int q = 0; q < T; ++q) sgp[q] = col_factor(tu, L.cd(q), h, unset);
-                mfma_product<T>(Pw, Ms, aR, aI, L, sgp);
-                qo = mfma_tdot<T>(Pw, aR, aI, L);  // Pw holds B^(a-1)
-                mfma_store<T>(Pw, aR, aI, L);
-                qe = mfma_tdot<T>(Pw, aR, aI, L);
-            }
-            cplx de = cmk(0.0, 0.0), dodd = cmk(0.0, 0.0);
-            if (LOOP) {
-                const cplx* zprev = Zb + ((a - 1) & 1) * NP;
-                cplx* zcur = Zb + (a & 1) * NP;
-                // z_a by row: lanes of one row sum their columns
-#pragma unroll
-                for (int p = 0; p < T; ++p) {
-                    cplx zs = cmk(0.0, 0.0);
-#pragma unroll
-                    for (int q = 0; q < T; ++q) cmac(zs, cmk(aR[p][q], aI[p][q]), Dv[L.cd(q)]);
-                    zs = cmk(sum_x_bc(zs.re), sum_x_bc(zs.im));
-                    if (L.x == 0 && L.bc == 0) zcur[L.rd(p)] = zs;
-                }
-                // y_a partial over the lane's rows, dotted with z_(a-1) and z_a at its columns
-                cplx xdp[T];
-#pragma unroll
-                for (int p = 0; p < T; ++p) xdp[p] = cmulr(XD0[L.rd(p)], col_factor(tu, L.rd(p), h, unset));
-#pragma unroll
-                for (int q = 0; q < T; ++q) {
-                    cplx ys = cmk(0.0, 0.0);
-#pragma unroll
-                    for (int p = 0; p < T; ++p) cmac(ys, xdp[p], cmk(aR[p][q], aI[p][q]));
-                    cmac(de, ys, zprev[L.cd(q)]);
-                    cmac(dodd, ys, zcur[L.cd(q)]);
-                }
-            }
-            const double vals[8] = {qo.re, qo.im, qe.re, qe.im, de.re, de.im, dodd.re, dodd.im};
-            const double tot = wave_sum8(vals, lane);
-            if ((lane & 7) == 0) {
-                // lane 8k holds value k: p_ko, p_ke, v_2a, v_(2a+1) (re, im)
-                const int k = lane >> 3;
-                const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
-                double* dst = k < 2 ? &pt[ko].re : k < 4 ? &pt[ke].re : k < 6 ? &lv[2 * a].re : &lv[2 * a + 1].re;
-                if (k >= 4 && !LOOP) dst = nullptr;
-                if (dst) dst[k & 1] = tot;
-            }
-        }
-#endif
         if (LOOP) {
             // v_1 = XD_d . D
             cplx part = cmk(0.0, 0.0);
@@ -692,7 +609,7 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
 
 }  // namespace
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool YS>
 __global__ void __launch_bounds__(256)
 wh_sieve_pm1m(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
               double4* __restrict__ partials, int64_t block_offset) {
@@ -703,7 +620,7 @@ wh_sieve_pm1m(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
     }
-    pm1m_body<T, LOOP>(jobs[lo], pool, partials, b, wh_smem);
+    pm1m_body<T, LOOP, YS>(jobs[lo], pool, partials, b, wh_smem);
 }
 
 // LDS of the matrix-core kernel: M and one power per wave (as the vector kernel), the
@@ -715,31 +632,31 @@ size_t wh_pm1m_lds_bytes(int T, int nw) {
     return (1 + nw) * mat + 2 * NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool YS>
 static hipError_t launch_m(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
     size_t lds = wh_pm1m_lds_bytes(T, nw);
-    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1m<T, LOOP>,
+    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1m<T, LOOP, YS>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((wh_sieve_pm1m<T, LOOP>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream, jobs,
+    hipLaunchKernelGGL((wh_sieve_pm1m<T, LOOP, YS>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream, jobs,
                        njobs, (const cplx*)pool, (double4*)partials, block_offset);
     return hipGetLastError();
 }
 
-hipError_t wh_launch_pm1m(int T, bool loop, const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                          int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
-#define WH_PM1M_CASE(TT)                                                                              \
-    case TT:                                                                                          \
-        return loop ? launch_m<TT, true>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream) \
-                    : launch_m<TT, false>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+hipError_t wh_launch_pm1m(int T, bool loop, bool mtl, const wh_job_dev* jobs, int njobs, const void* pool,
+                          void* partials, int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+#define WH_PM1M_CASE(TT)                                                                                   \
+    case TT:                                                                                               \
+        if (!loop) return launch_m<TT, false, false>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream); \
+        if (mtl) return launch_m<TT, true, true>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);     \
+        return launch_m<TT, true, false>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
     switch (T) {
         WH_PM1M_CASE(1)
         WH_PM1M_CASE(2)
         WH_PM1M_CASE(3)
         WH_PM1M_CASE(4)
         WH_PM1M_CASE(5)
-        WH_PM1M_CASE(6)
         default: return hipErrorInvalidValue;
     }
 #undef WH_PM1M_CASE
