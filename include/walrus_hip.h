@@ -117,6 +117,9 @@ typedef struct wh_request {
 /* wh_request.options: on the all-ones kernel with complex inputs, run the products on the
  * vector FMA units (register-tiled) instead of the FP64 matrix cores (the default). */
 #define WH_OPT_VALU 8
+/* wh_request.options: on the matrix cores, form the full powers B^a = B^(a-1) B for hafnians
+ * instead of the symmetric powers C_a = B^a X diag(d) (upper triangle only, the default). */
+#define WH_OPT_NONSYM 16
 /* wh_request.options: a montrealer request (montrealer / lmontrealer, _montrealer.py:37-101)
  * over n = E modes.  a_off: X Sigma (2E x 2E, rows of Sigma swapped by halves); for the loop
  * montrealer d_off: X zeta and v_off: conj(X zeta), else both -1.  edge_reps, glynn and

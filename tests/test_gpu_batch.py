@@ -137,24 +137,29 @@ def test_half_and_full_products(loop, real):
 
 @pytest.mark.parametrize("loop", [False, True])
 def test_matrix_cores_and_vector_units(loop):
-    """The complex all-ones kernel on the FP64 matrix cores (default) and on the vector units,
-    tile sides 1..6 (n = 8 .. 48), against truth; bar as above."""
+    """The complex all-ones kernel on the FP64 matrix cores with symmetric powers (default), with
+    full powers, and on the vector units, tile sides 1..6 (n = 8 .. 48), against truth; bar as
+    above."""
     rng = np.random.default_rng(61 + loop)
     sizes = [4, 6, 8, 12, 16, 20, 24, 30, 32, 36, 40, 44, 48] if not loop else [6, 8, 14, 16, 22, 24, 30, 32, 38, 40, 46]
     reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in sizes]
     preps = [B.prepare(A, loop=l) for A, l in reqs]
-    mc = B.DeviceBatch(preps)
-    vu = B.DeviceBatch(preps, valu_products=True)
-    mc.run()
-    vu.run()
-    rm, rv = mc.results(), vu.results()
-    for (A, l), a, v in zip(reqs, rm, rv):
+    runs = {"symmetric powers": B.DeviceBatch(preps), "full powers": B.DeviceBatch(preps, full_powers=True),
+            "vector units": B.DeviceBatch(preps, valu_products=True)}
+    res = {}
+    for name, db in runs.items():
+        print("all-ones kernel: %s, loop=%s" % (name, loop), flush=True)
+        db.run()
+        res[name] = db.results()
+    for k, (A, l) in enumerate(reqs):
         n = A.shape[0]
         if n <= 24:
             truth = O.hafnian_sieve(A, loop=l, precision="dd")
             ref64 = O.hafnian_sieve(A, loop=l, precision="double")
             bar = max(TOL, 2 * relerr(ref64, truth))
-            assert relerr(a, truth) < bar, ("matrix cores", n, relerr(a, truth), bar)
-            assert relerr(v, truth) < bar, ("vector units", n, relerr(v, truth), bar)
-        else:  # beyond the oracle's seconds: the two kernels agree to the bar
-            assert relerr(a, v) < 1e-10, (n, relerr(a, v))
+            for name in runs:
+                assert relerr(res[name][k], truth) < bar, (name, n, relerr(res[name][k], truth), bar)
+        else:  # beyond the oracle's seconds: the kernels agree to the bar
+            v = res["vector units"][k]
+            for name in ("symmetric powers", "full powers"):
+                assert relerr(res[name][k], v) < 1e-10, (name, n, relerr(res[name][k], v))

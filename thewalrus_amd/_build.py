@@ -15,7 +15,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libwalrus_hip.so")
 OBJDIR = os.path.join(HERE, "_obj")
 
-SOURCES = ["capi.hip", "sieve_generic.hip", "sieve_pm1.hip", "sieve_pm1m.hip", "sieve_lhb.hip", "hermite.hip"]
+SOURCES = ["capi.hip", "sieve_generic.hip", "sieve_pm1.hip", "sieve_pm1m.hip", "sieve_pm1s.hip", "sieve_lhb.hip", "hermite.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

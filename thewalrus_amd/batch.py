@@ -109,10 +109,12 @@ class DeviceBatch:
             (h - 1 products per term) instead of the half-products formulation
         valu_products (bool): on the all-ones kernel with complex inputs, run the products on
             the vector FMA units instead of the FP64 matrix cores
+        full_powers (bool): on the matrix cores, form the full powers B^a instead of the
+            symmetric powers B^a X diag(d) (upper triangle only)
     """
 
     def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False,
-                 valu_products=False):
+                 valu_products=False, full_powers=False):
         import torch
 
         self.torch = torch
@@ -133,7 +135,8 @@ class DeviceBatch:
         self.plan = None
         if self.nreq == 0:
             return
-        options = (1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
+        options = ((1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
+                   | (16 if full_powers else 0))
         reqs, pool = pack_jobs(jobs, options=options)
         with torch.cuda.device(self.device):
             self.pool = torch.from_numpy(pool.view(np.float64).copy()).to(self.device)

@@ -167,7 +167,8 @@ int describe(const wh_request& r, wh_job_dev& j) {
 
 // Kernel families.  PM1: Glynn, even N, every multiplicity 1 (m = n for every term); the
 // register-tiled kernel with T x T tiles per lane, n_pad = 8T.  GENERIC: everything else.
-enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2 };
+enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2, KIND_PM1S = 3 };
+static bool all_ones_kind(int k) { return k == KIND_PM1 || k == KIND_PM1M || k == KIND_PM1S; }
 constexpr int kPm1MaxT = 6;
 
 struct GroupKey {
@@ -186,11 +187,14 @@ struct GroupKey {
 };
 
 // Complex inputs with the half-products formulation run the products on the matrix cores
-// (KIND_PM1M) unless WH_OPT_VALU asks for the register-tiled vector kernel.
+// unless WH_OPT_VALU asks for the register-tiled vector kernel: hafnians with the symmetric
+// powers C_a = B^a X D_d (KIND_PM1S, upper block groups only), the montrealer (whose 0/1
+// factors break that symmetry) and WH_OPT_NONSYM requests with the full powers (KIND_PM1M).
 static void pick_matrix_cores(GroupKey& g, int options) {
     // T = 6 (n in [42, 48]) keeps 72 complex accumulators per lane and spills on the matrix
     // cores: it stays on the vector kernel
-    if (g.kind == KIND_PM1 && !g.real && g.half && g.T <= 5 && !(options & WH_OPT_VALU)) g.kind = KIND_PM1M;
+    if (g.kind == KIND_PM1 && !g.real && g.half && g.T <= 5 && !(options & WH_OPT_VALU))
+        g.kind = (g.mtl || (options & WH_OPT_NONSYM)) ? KIND_PM1M : KIND_PM1S;
 }
 
 GroupKey classify(const wh_job_dev& j, int options) {
@@ -223,11 +227,13 @@ GroupKey classify(const wh_job_dev& j, int options) {
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
 // under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
-int pm1_waves(int T, bool real, bool matrix_cores) {
+int pm1_waves(int T, bool real, int kind) {
     const size_t lds_cap = 160 * 1024;
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
-        size_t b = matrix_cores ? wh_pm1m_lds_bytes(T, nw) : wh_pm1_lds_bytes(T, real, nw);
+        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw)
+                   : kind == KIND_PM1M ? wh_pm1m_lds_bytes(T, nw)
+                                       : wh_pm1_lds_bytes(T, real, nw);
         if (b > lds_cap) break;
         int per_cu = (int)(lds_cap / b) * nw;
         if (per_cu > 16) per_cu = 16;
@@ -237,7 +243,7 @@ int pm1_waves(int T, bool real, bool matrix_cores) {
 }
 
 int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
-    if (g.kind == KIND_PM1 || g.kind == KIND_PM1M) {
+    if (all_ones_kind(g.kind)) {
         int64_t c = j.steps / 1024;
         if (c < 4 * nw) c = 4 * nw;
         if (c > 4096) c = 4096;
@@ -255,13 +261,15 @@ double term_flops(const wh_job_dev& j, const GroupKey& g) {
     // algorithmic FP64 flops of the products per term, as the kernels run them:
     // PM1: h - 1 products of n x n (8 flops per complex multiply-add, 2 per real one), or
     // ceil(h/2) - 1 with the half-products formulation;
-    // GENERIC: (min(h + 1, m) - 2) products of m x m with m = 2 * nnz(delta) <= 2E (upper bound)
+    // GENERIC: (min(h + 1, m) - 2) products of m x m with m = 2 * nnz(delta) <= 2E (upper bound);
+    // PM1S: the symmetric powers need the m (m + 1) / 2 entries of the upper triangle only
     int h = j.N / 2;
     int m = 2 * j.E;
     int np = std::min(h + 1, m) - 2;
-    if ((g.kind == KIND_PM1 || g.kind == KIND_PM1M) && g.half) np = (h + 1) / 2 - 1;
+    if (all_ones_kind(g.kind) && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
     double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;
+    if (g.kind == KIND_PM1S) return c * (m * (m + 1.0) / 2.0) * m * np;
     return c * m * (double)m * m * np;
 }
 
@@ -307,8 +315,7 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
             gr.key = g;
             gr.job_begin = k;
             gr.block_begin = base;
-            gr.nw = (g.kind == KIND_PM1 || g.kind == KIND_PM1M)
-                        ? pm1_waves(g.T, g.real != 0, g.kind == KIND_PM1M) : 1;
+            gr.nw = all_ones_kind(g.kind) ? pm1_waves(g.T, g.real != 0, g.kind) : 1;
             L.groups.push_back(gr);
         }
         Group& gr = L.groups.back();
@@ -337,7 +344,9 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
         int nj = (int)(g.job_end - g.job_begin);
         int64_t nb = g.block_end - g.block_begin;
         hipError_t e;
-        if (g.key.kind == KIND_PM1M)
+        if (g.key.kind == KIND_PM1S)
+            e = wh_launch_pm1s(g.key.T, g.key.loop != 0, gj, nj, pool, partials, nb, g.block_begin, g.nw, s);
+        else if (g.key.kind == KIND_PM1M)
             e = wh_launch_pm1m(g.key.T, g.key.loop != 0, g.key.mtl != 0, gj, nj, pool, partials, nb, g.block_begin,
                                g.nw, s);
         else if (g.key.kind == KIND_PM1)

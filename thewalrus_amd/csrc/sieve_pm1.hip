@@ -105,10 +105,12 @@ template <int T> struct Lds {
 
 template <int T, class E>
 __device__ __forceinline__ void tile_store(E* Pw, const E (&acc)[T][T], int I0, int J0) {
+    wave_lds_sync();  // the other lanes' reads of the previous power are done
 #pragma unroll
     for (int r = 0; r < T; ++r)
 #pragma unroll
         for (int c = 0; c < T; ++c) Pw[(I0 + r) * Lds<T>::S + J0 + c] = acc[r][c];
+    wave_lds_sync();  // the whole power is stored before any lane reads it
 }
 
 // acc = (Pw Ms) diag(sign_J): one step of the power sequence, k summed in order.  The
@@ -429,6 +431,7 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             cplx v1 = wave_csum(etoc(part));
             if (lane == 0) lv[1] = v1;
         }
+        wave_lds_sync();  // pt and lv as lane 0 stored them
 
         if (mtl) {
             // montrealer term: sign (p_h / 2h + v_h / 2), v_h = conj(zeta_S) Sigma_S^(h-1) zeta_S

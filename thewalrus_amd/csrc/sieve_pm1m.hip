@@ -90,12 +90,12 @@ __device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
 // b_i = i a_i = p_i / 2 (+ i v_i / 2 with loops), the exponential's recurrence; two partial
 // sums per k halve the dependent chain.
 template <int H, bool LOOP>
-__device__ __forceinline__ cplx coeff_top(const cplx* pt, const cplx* lv, int h) {
+__device__ __forceinline__ cplx coeff_top(const cplx* pt, const cplx* lv, cplx v1, int h) {
     cplx b[H + 1], e[H + 1];
 #pragma unroll
     for (int i = 1; i <= H; ++i) {
         cplx v = cmulr(pt[i], 0.5);
-        if (LOOP) v = cadd(v, cmulr(lv[i], 0.5 * i));
+        if (LOOP) v = cadd(v, cmulr(i == 1 ? v1 : lv[i], 0.5 * i));
         b[i] = i <= h ? v : cmk(0.0, 0.0);
     }
     e[0] = cmk(1.0, 0.0);
@@ -276,6 +276,7 @@ __device__ __forceinline__ void epi_step(int kc, int st, EpiPipe<T>& E, const do
     }
     if (kc != NK - 1) return;
     if (LOOP) {
+        if (st == 0) wave_lds_sync();  // the z rows of every lane are stored
         if (YS) {
             if (st == 0) {
 #pragma unroll
@@ -523,6 +524,7 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
         if (LOOP) {
             for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];
         }
+        wave_lds_sync();
         const int Ahalf = (h + 1) / 2;
         // powers up to B^A, A = ceil(h/2): tr(B^(2a)) = <B^a, (B^a)^T>, tr(B^(2a+1)) =
         // <B^(a+1), (B^a)^T>; loop scalars v_(2a) = y_a . z_(a-1), v_(2a+1) = y_a . z_a with
@@ -530,6 +532,7 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
         // the epilogue of B^(a-1); the accumulators alternate between two register sets.
         {
             mfma_store<T>(Pw, aR, aI, L);
+            wave_lds_sync();
             cplx qo = cmk(0.0, 0.0);
             if (L.diag()) {
 #pragma unroll
@@ -546,12 +549,16 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
                 mfma_product_epi<T, LOOP, YS>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
                                           Zb + (a & 1) * NP, Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, bR, bI, L);
+                wave_lds_sync();
                 mfma_store<T>(Pw, bR, bI, L);
+                wave_lds_sync();
                 // B^(a+1) into a, epilogue of B^a from b
                 mfma_product_epi<T, LOOP, YS>(Pw, Ms, aR, aI, L, sgp, bR, bI, a, qo, lane, Dv, XD0,
                                           Zb + ((a + 1) & 1) * NP, Zb + (a & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, aR, aI, L);
+                wave_lds_sync();
                 mfma_store<T>(Pw, aR, aI, L);
+                wave_lds_sync();
             }
             if (a <= Ahalf) {  // one power left: B^a into b, then its epilogue alone
                 double sgp[T];
@@ -560,7 +567,9 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
                 mfma_product_epi<T, LOOP, YS>(Pw, Ms, bR, bI, L, sgp, aR, aI, a - 1, qo, lane, Dv, XD0,
                                           Zb + (a & 1) * NP, Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
                 qo = mfma_tdot<T>(Pw, bR, bI, L);
+                wave_lds_sync();
                 mfma_store<T>(Pw, bR, bI, L);
+                wave_lds_sync();
                 epi_alone<T, LOOP, YS>(bR, bI, a, qo, L, lane, Pw, Dv, XD0, Zb + ((a - 1) & 1) * NP,
                                    Zb + (a & 1) * NP, tu, h, unset, pt, lv);
             } else {  // the last power (a - 1) is in a
@@ -568,19 +577,20 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
                                    Zb + ((a - 1) & 1) * NP, tu, h, unset, pt, lv);
             }
         }
+        cplx v1 = cmk(0.0, 0.0);
         if (LOOP) {
-            // v_1 = XD_d . D
+            // v_1 = XD_d . D (every lane holds the sum)
             cplx part = cmk(0.0, 0.0);
             for (int e = lane; e < n; e += WH_WAVE) cmac(part, cmulr(XD0[e], col_factor(tu, e, h, unset)), Dv[e]);
-            cplx v1 = wave_csum(part);
-            if (lane == 0) lv[1] = v1;
+            v1 = wave_csum(part);
         }
+        wave_lds_sync();  // pt and lv as the epilogues stored them
 
         if (mtl) {
             // montrealer term: sign (p_h / 2h + v_h / 2)
             if (lane == h) {
                 cplx f = cdivr(pt[h], (double)(2 * h));
-                if (LOOP) f = cadd(f, cmulr(lv[h], 0.5));
+                if (LOOP) f = cadd(f, cmulr(h == 1 ? v1 : lv[h], 0.5));
                 acc_sum.add(cmulr(f, prefac));
             }
             continue;
@@ -589,7 +599,7 @@ __device__ __forceinline__ void pm1m_body(const wh_job_dev& J, const cplx* __res
         // reference builds it factor by factor (_hafnian.py:200-208, :232-241); here every
         // lane runs the power-series exponential e_k = (1/k) sum_(i<=k) (i a_i) e_(k-i) in
         // registers (the same polynomial, no LDS traffic on the wave's serial tail)
-        const cplx top = coeff_top<4 * T, LOOP>(pt, lv, h);
+        const cplx top = coeff_top<4 * T, LOOP>(pt, lv, v1, h);
         if (lane == h) acc_sum.add(cmulr(top, prefac));
     }
 

@@ -1,0 +1,546 @@
+// sieve_pm1s.hip -- the all-ones Glynn sieve on the FP64 matrix cores with symmetric powers:
+// the terms of _hafnian.py:443-462 (plain) and :546-569 (even loop) when rpt is all ones,
+// complex matrices, half-products formulation, with 60% of the matrix-core work of
+// sieve_pm1m.hip at n_pad = 40.
+//
+// With A = Ax symmetric, X the swap of the two halves and D_d = diag(d) (d = +-1 repeating over
+// the halves, so X D_d = D_d X and (X D_d)^2 = I), the term matrix is B = A X D_d and
+//     C_a = B^a X D_d = A (X D_d A)^(a-1)   is symmetric.
+// The kernel keeps K_a = C_a D_d = B^a X (K_a[i][k] = C_a[i][k] d_k):
+//     K_1 = A D_d,   K_(a+1) = K_a (X A) D_d,
+// the product of sieve_pm1m.hip with X A for A X, formed for the upper block groups only
+// (T (T + 1) / 2 of the T^2 groups); the LDS copy is completed from the symmetry of C,
+// K[k][i] = d_i d_k K[i][k].  The power traces need no signs:
+//     tr(B^(a+b)) = sum_(i,k) d_i d_k C_a[i][k] C_b[swap k][swap i]
+//                 = sum_(i,k) K_a[i][k] K_b[swap k][swap i],
+// a summand symmetric in (i, k), so the upper groups off the diagonal count twice.  The loop
+// vectors z_a = B^a D follow the chain z_a = A (X D_d z_(a-1)), and the loop dots use
+// y_a = X D_d z_a as in sieve_pm1m.hip.  The montrealer (0/1 factors: D_d^2 is not I) stays on
+// sieve_pm1m.hip.
+//
+// Lane map and MFMA layout as sieve_pm1m.hip: x = lane & 3, br = bit 2, bc = bit 3,
+// y = lane >> 4; group (p, q) holds K[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
+// rows and columns live in one mask register and flip the sign bit of the operands they scale.
+#include "wh_device.h"
+#include "wh_launch.h"
+
+namespace {
+
+#ifndef WH_PM1S_PAD
+#define WH_PM1S_PAD 2
+#endif
+
+static __constant__ double kInvJs[WH_EMAX + 1] = {
+    0.0,      1.0,      1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,  1.0 / 8,
+    1.0 / 9,  1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+    1.0 / 18, 1.0 / 19, 1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26,
+    1.0 / 27, 1.0 / 28, 1.0 / 29, 1.0 / 30, 1.0 / 31, 1.0 / 32};
+
+__device__ __forceinline__ double mfma4(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// d_c of the current term: +1 if digit (c mod h) is set, -1 otherwise, 1 on the padding
+__device__ __forceinline__ double sgn(uint64_t t, int c, int h) {
+    int i = c < h ? c : c - h;
+    if (i >= h) return 1.0;
+    return ((t >> (h - 1 - i)) & 1ull) ? 1.0 : -1.0;
+}
+// 1 where d_c = -1 (digit unset), 0 where d_c = +1 or c is padding
+__device__ __forceinline__ uint32_t negbit(uint64_t t, int c, int h) {
+    int i = c < h ? c : c - h;
+    if (i >= h) return 0u;
+    return ((t >> (h - 1 - i)) & 1ull) ? 0u : 1u;
+}
+// v times (-1)^bit, bit in {0, 1}: the sign bit of the high word
+__device__ __forceinline__ double sflip(double v, uint32_t bit) {
+    return __hiloint2double(__double2hiint(v) ^ (int)(bit << 31), __double2loint(v));
+}
+__device__ __forceinline__ cplx cflip(cplx v, uint32_t bit) { return cmk(sflip(v.re, bit), sflip(v.im, bit)); }
+// the partner of index c in the other half (identity on the padding)
+__device__ __forceinline__ int swp(int c, int h) { return c < h ? c + h : (c < 2 * h ? c - h : c); }
+
+__device__ __forceinline__ double dswap32(double a, double b) {
+    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ double dswap16(double a, double b) {
+    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+template <int T> struct GeoS {
+    static constexpr int NP = 8 * T;
+    static constexpr int S = NP + WH_PM1S_PAD;
+};
+
+struct LaneMapS {
+    int x, y, br, bc;
+    __device__ int rd(int p) const { return 8 * p + 4 * br + y; }
+    __device__ int cd(int q) const { return 8 * q + 4 * bc + x; }
+};
+
+// [eta^h] exp(sum_i a_i eta^i) by the exponential recurrence (as sieve_pm1m.hip)
+template <int H, bool LOOP>
+__device__ __forceinline__ cplx coeff_top_s(const cplx* pt, const cplx* lv, cplx v1, int h) {
+    cplx b[H + 1], e[H + 1];
+#pragma unroll
+    for (int i = 1; i <= H; ++i) {
+        cplx v = cmulr(pt[i], 0.5);
+        if (LOOP) v = cadd(v, cmulr(i == 1 ? v1 : lv[i], 0.5 * i));
+        b[i] = i <= h ? v : cmk(0.0, 0.0);
+    }
+    e[0] = cmk(1.0, 0.0);
+    cplx res = e[0];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        cplx s0 = cmk(0.0, 0.0), s1 = cmk(0.0, 0.0);
+#pragma unroll
+        for (int i = 1; i <= k; i += 2) cmac(s0, b[i], e[k - i]);
+#pragma unroll
+        for (int i = 2; i <= k; i += 2) cmac(s1, b[i], e[k - i]);
+        e[k] = cmulr(cadd(s0, s1), kInvJs[k]);
+        if (k == h) res = e[k];
+    }
+    return res;
+}
+
+// lane part of sum_(i,k) Ka[i][k] Kb[swap k][swap i] over the upper groups (off-diagonal twice)
+template <int T>
+__device__ __forceinline__ cplx sym_tdot(const cplx* Kb, const double (&aR)[T][T], const double (&aI)[T][T],
+                                         const LaneMapS& L, int h) {
+    constexpr int S = GeoS<T>::S;
+    cplx sd = cmk(0.0, 0.0), so = cmk(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < T; ++p)
+#pragma unroll
+        for (int q = p; q < T; ++q) {
+            const cplx x = Kb[swp(L.cd(q), h) * S + swp(L.rd(p), h)];
+            if (q == p) cmac(sd, cmk(aR[p][q], aI[p][q]), x);
+            else cmac(so, cmk(aR[p][q], aI[p][q]), x);
+        }
+    return cadd(sd, cmulr(so, 2.0));
+}
+
+// K into LDS: each upper group at its place and, off the diagonal, at the mirrored place with
+// the sign d_i d_k (bits p and T + q of the lane's sign mask)
+template <int T>
+__device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], const double (&aI)[T][T],
+                                          const LaneMapS& L, uint32_t fm) {
+    constexpr int S = GeoS<T>::S;
+#pragma unroll
+    for (int p = 0; p < T; ++p)
+#pragma unroll
+        for (int q = p; q < T; ++q) {
+            const cplx v = cmk(aR[p][q], aI[p][q]);
+            X[L.rd(p) * S + L.cd(q)] = v;
+            if (p < q) X[L.cd(q) * S + L.rd(p)] = cflip(v, ((fm >> p) ^ (fm >> (T + q))) & 1u);
+        }
+}
+
+// Epilogue micro-steps of power a (interleaved with the next product as in sieve_pm1m.hip): the
+// even chunk of row group r takes the row's share of the trace sum and of z_a = A t
+// (t = X D_d z_(a-1)), the odd chunk reduces and stores the row's z, the last chunk forms the
+// loop dots and sums the eight results over the wave.
+template <int T>
+struct EpiS {
+    cplx qe, qf, de, dodd, zs;
+    cplx tx[T], av[T], tv[T], zc[3];
+    double s1[4], s2[2], s3;
+};
+
+template <int T, bool LOOP>
+__device__ __forceinline__ int epis_steps(int kc) {
+    constexpr int NK = 2 * T;
+    const int r = kc >> 1;
+    int n = 0;
+    if ((kc & 1) == 0) n = (T - r) + (LOOP ? T : 0);
+    else n = LOOP ? 4 : 0;
+    if (kc == NK - 1) n += (LOOP ? 2 : 0) + 12;
+    return n;
+}
+
+template <int T, bool LOOP>
+__device__ __forceinline__ void epis_init(EpiS<T>& E, const LaneMapS& L, const cplx* zprev, uint32_t fm,
+                                          int h) {
+    E.qe = E.qf = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
+    if (LOOP) {  // t = X D_d z_(a-1) at the lane's columns
+#pragma unroll
+        for (int q = 0; q < T; ++q) E.tv[q] = cflip(zprev[swp(L.cd(q), h)], (fm >> (T + q)) & 1u);
+    }
+}
+
+// the operands of row group r's micro-steps
+template <int T, bool LOOP>
+__device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS& L, const cplx* Kb,
+                                              const cplx* Ms, int h) {
+    constexpr int S = GeoS<T>::S;
+    const int rs = swp(L.rd(r), h);
+#pragma unroll
+    for (int q = r; q < T; ++q) E.tx[q] = Kb[swp(L.cd(q), h) * S + rs];
+    if (LOOP) {  // A[rd(r)][cd(q)] = (X A)[swap rd(r)][cd(q)]
+#pragma unroll
+        for (int q = 0; q < T; ++q) E.av[q] = Ms[rs * S + L.cd(q)];
+    }
+}
+
+template <int T, bool LOOP>
+__device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const double (&oR)[T][T],
+                                          const double (&oI)[T][T], const LaneMapS& L, int lane, int a_prev,
+                                          cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt, cplx* lv,
+                                          uint64_t tu, int h) {
+    constexpr int NK = 2 * T;
+    const int r = kc >> 1;
+    if ((kc & 1) == 0) {
+        if (st < T - r) {
+            const int q = r + st;
+            if (q == r) cmac(E.qe, cmk(oR[r][q], oI[r][q]), E.tx[q]);
+            else cmac(E.qf, cmk(oR[r][q], oI[r][q]), E.tx[q]);
+            return;
+        }
+        if (LOOP && st < T - r + T) {
+            const int q = st - (T - r);
+            cmac(E.zs, E.av[q], E.tv[q]);
+            return;
+        }
+        return;
+    }
+    if (LOOP) {
+        if (st == 0) { E.zs.re += dpp_d<WH_DPP_XOR1>(E.zs.re); E.zs.im += dpp_d<WH_DPP_XOR1>(E.zs.im); return; }
+        if (st == 1) { E.zs.re += dpp_d<WH_DPP_XOR2>(E.zs.re); E.zs.im += dpp_d<WH_DPP_XOR2>(E.zs.im); return; }
+        if (st == 2) { E.zs.re += dpp_d<WH_DPP_ROR8>(E.zs.re); E.zs.im += dpp_d<WH_DPP_ROR8>(E.zs.im); return; }
+        if (st == 3) { zcur[L.rd(r)] = E.zs; E.zs = cmk(0.0, 0.0); return; }  // same value from the row's 8 lanes
+        st -= 4;
+    }
+    if (kc != NK - 1) return;
+    if (LOOP) {
+        // lane e: d_e z_a[swap e] times z_(a-1)[e] and z_a[e]
+        const int n = 2 * h;
+        const int e = lane < n ? lane : 0;
+        if (st == 0) {
+            wave_lds_sync();  // the z rows of every lane are stored
+            E.zc[0] = zcur[e];
+            E.zc[1] = zprev[e];
+            E.zc[2] = zcur[swp(e, h)];
+            return;
+        }
+        if (st == 1) {
+            const cplx f = cmulr(E.zc[2], lane < n ? sgn(tu, e, h) : 0.0);
+            E.de = cmul(f, E.zc[1]);
+            E.dodd = cmul(f, E.zc[0]);
+            return;
+        }
+        st -= 2;
+    }
+    if (st == 0) { E.qe = cadd(E.qe, cmulr(E.qf, 2.0)); return; }
+    st -= 1;
+    const double v[8] = {qo_prev.re, qo_prev.im, E.qe.re, E.qe.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
+    if (st < 4) { E.s1[st] = dswap32(v[st], v[st + 4]); return; }
+    if (st < 6) { E.s2[st - 4] = dswap16(E.s1[st - 4], E.s1[st - 2]); return; }
+    if (st == 6) {
+        const bool b3 = (lane >> 3) & 1;
+        double keep = b3 ? E.s2[1] : E.s2[0], send = b3 ? E.s2[0] : E.s2[1];
+        E.s3 = keep + dpp_d<WH_DPP_ROR8>(send);
+        return;
+    }
+    if (st == 7) { E.s3 += dpp_d<WH_DPP_XOR1>(E.s3); return; }
+    if (st == 8) { E.s3 += dpp_d<WH_DPP_XOR2>(E.s3); return; }
+    if (st == 9) { E.s3 += dpp_d<WH_DPP_HALF_MIRROR>(E.s3); return; }
+    if (st == 10) {
+        // lane 8k holds value k: p_ko, p_ke, v_2a, v_(2a+1) (re, im); the other lanes write the
+        // spare slot pt[WH_EMAX + 1] (never read)
+        const int k = lane >> 3;
+        const int a = a_prev;
+        const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+        double* dst = k < 2 ? &pt[ko].re : k < 4 ? &pt[ke].re : k < 6 ? &lv[2 * a].re : &lv[2 * a + 1].re;
+        dst += k & 1;
+        if ((lane & 7) != 0) dst = &pt[WH_EMAX + 1].re;
+        *dst = E.s3;
+    }
+}
+
+// Kn = Kb (X A) D_d for the upper groups (fully unrolled k chunks) with power a_prev's epilogue
+// (from Ko and Kb = Pw = K_(a_prev)) between the MFMA pairs; operands of chunk kc + 1 load
+// while chunk kc runs.
+template <int T, bool LOOP>
+__device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Kb, const cplx* __restrict__ Ms,
+                                                double (&nR)[T][T], double (&nI)[T][T], const LaneMapS& L,
+                                                uint32_t fm, const double (&oR)[T][T], const double (&oI)[T][T],
+                                                int a_prev, cplx qo_prev, int lane, const cplx* zprev,
+                                                cplx* zcur, uint64_t tu, int h, cplx* pt, cplx* lv) {
+    constexpr int S = GeoS<T>::S;
+    constexpr int NK = 2 * T;
+#pragma unroll
+    for (int p = 0; p < T; ++p)
+#pragma unroll
+        for (int q = p; q < T; ++q) {
+            nR[p][q] = 0.0;
+            nI[p][q] = 0.0;
+        }
+    EpiS<T> E;
+    epis_init<T, LOOP>(E, L, zprev, fm, h);
+    const cplx* pa = Kb + (4 * L.br + L.x) * S + L.y;
+    const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
+    cplx avn[T], bqn[T];
+#pragma unroll
+    for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S];
+#pragma unroll
+    for (int q = 0; q < T; ++q) bqn[q] = pb[8 * q];
+    epis_load_row<T, LOOP>(0, E, L, Kb, Ms, h);
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+        cplx av[T], bq[T];
+        double nbi[T];
+#pragma unroll
+        for (int p = 0; p < T; ++p) av[p] = avn[p];
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            bq[q] = cflip(bqn[q], (fm >> (T + q)) & 1u);  // column factor d_j
+            nbi[q] = -bq[q].im;
+        }
+        if (kc + 1 < NK) {
+#pragma unroll
+            for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S + 4 * (kc + 1)];
+#pragma unroll
+            for (int q = 0; q < T; ++q) bqn[q] = pb[4 * (kc + 1) * S + 8 * q];
+        }
+        if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP>((kc + 1) >> 1, E, L, Kb, Ms, h);
+        const int nst = epis_steps<T, LOOP>(kc);
+        int st = 0;
+#pragma unroll
+        for (int p = 0; p < T; ++p)
+#pragma unroll
+            for (int q = p; q < T; ++q) {
+                nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
+                nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
+                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                ++st;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+        for (int p = 0; p < T; ++p)
+#pragma unroll
+            for (int q = p; q < T; ++q) {
+                nR[p][q] = mfma4(av[p].im, nbi[q], nR[p][q]);
+                nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
+                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                ++st;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        for (; st < nst; ++st)
+            epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+    }
+}
+
+// the last power's epilogue, the same micro-steps in sequence
+template <int T, bool LOOP>
+__device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const double (&aI)[T][T], int a, cplx qo,
+                                           const LaneMapS& L, uint32_t fm, int lane, const cplx* Kb,
+                                           const cplx* Ms, const cplx* zprev, cplx* zcur, uint64_t tu, int h,
+                                           cplx* pt, cplx* lv) {
+    constexpr int NK = 2 * T;
+    EpiS<T> E;
+    epis_init<T, LOOP>(E, L, zprev, fm, h);
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+        if ((kc & 1) == 0) epis_load_row<T, LOOP>(kc >> 1, E, L, Kb, Ms, h);
+        const int nst = epis_steps<T, LOOP>(kc);
+#pragma unroll
+        for (int st = 0; st < 2 * T + 18; ++st)
+            if (st < nst) epis_step<T, LOOP>(kc, st, E, aR, aI, L, lane, a, qo, zprev, zcur, pt, lv, tu, h);
+    }
+}
+
+template <int T, bool LOOP>
+__device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __restrict__ pool,
+                                          double4* __restrict__ partials, int64_t b, unsigned char* smem) {
+    constexpr int NP = GeoS<T>::NP;
+    constexpr int S = GeoS<T>::S;
+    const int nw = blockDim.x / WH_WAVE;
+    const int wave = threadIdx.x / WH_WAVE;
+    const int lane = threadIdx.x % WH_WAVE;
+    LaneMapS L;
+    L.x = lane & 3;
+    L.br = (lane >> 2) & 1;
+    L.bc = (lane >> 3) & 1;
+    L.y = lane >> 4;
+    const int n = 2 * J.E;
+    const int h = J.E;
+
+    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one K per wave, D, per wave pt, lv, z rows x 2
+    cplx* Ms = reinterpret_cast<cplx*>(smem);
+    const size_t mat = ((size_t)NP * S * sizeof(cplx) + 31) / 32 * 32 / sizeof(cplx);
+    cplx* Pw = Ms + mat + wave * mat;
+    cplx* Dv = Ms + (1 + nw) * mat;
+    constexpr int SMALL = 2 * (WH_EMAX + 2) + 2 * NP;
+    cplx* small = Dv + NP + wave * SMALL;
+    cplx* pt = small;
+    cplx* lv = pt + (WH_EMAX + 2);
+    cplx* Zb = lv + (WH_EMAX + 2);  // z_a by row, two buffers (a even / odd)
+    double4* wres = reinterpret_cast<double4*>(Dv + NP + nw * SMALL);
+
+    // ---- stage X A (row r holds A's row swap(r)), zero padded, and D
+    const cplx* A = pool + J.a_off;
+    for (int e = threadIdx.x; e < NP * NP; e += blockDim.x) {
+        int r = e / NP, c = e - r * NP;
+        cplx v = cmk(0.0, 0.0);
+        if (r < n && c < n) v = A[(int64_t)swp(r, h) * n + c];
+        Ms[r * S + c] = v;
+    }
+    if (LOOP) {
+        const cplx* Dg = pool + J.d_off;
+        for (int e = threadIdx.x; e < NP; e += blockDim.x) Dv[e] = e < n ? Dg[e] : cmk(0.0, 0.0);
+    }
+    __syncthreads();
+
+    const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
+    int64_t t1 = t0 + J.chunk_terms;
+    if (t1 > J.steps) t1 = J.steps;
+
+    csum acc_sum;
+    acc_sum.zero();
+
+    for (int64_t t = t0 + wave; t < t1; t += nw) {
+        const uint64_t tu = (uint64_t)t;
+        const double prefac = ((h - __popcll(tu)) & 1) ? -1.0 : 1.0;
+        // the lane's signs: bit p for row rd(p), bit T + q for column cd(q)
+        uint32_t fm = 0;
+#pragma unroll
+        for (int p = 0; p < T; ++p) fm |= negbit(tu, L.rd(p), h) << p;
+#pragma unroll
+        for (int q = 0; q < T; ++q) fm |= negbit(tu, L.cd(q), h) << (T + q);
+
+        // K_1 = A D_d (upper groups), into LDS whole
+        double aR[T][T], aI[T][T];
+#pragma unroll
+        for (int p = 0; p < T; ++p)
+#pragma unroll
+            for (int q = p; q < T; ++q) {
+                const cplx m = cflip(Ms[swp(L.rd(p), h) * S + L.cd(q)], (fm >> (T + q)) & 1u);
+                aR[p][q] = m.re;
+                aI[p][q] = m.im;
+            }
+        sym_store<T>(Pw, aR, aI, L, fm);
+        // p_1 = tr(B) = sum_e d_e A[e][swap e] (lane partials, summed with power 1's results)
+        cplx qo = cmk(0.0, 0.0);
+        if (lane < n) qo = cmulr(Ms[swp(lane, h) * S + swp(lane, h)], sgn(tu, lane, h));
+        if (LOOP) {
+            for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];  // z_0 = D
+        }
+        wave_lds_sync();
+        // powers up to K_A, A = ceil(h/2), software pipelined as sieve_pm1m.hip: the product
+        // forming K_a runs with the epilogue of K_(a-1); the accumulators alternate between two
+        // register sets
+        const int Ahalf = (h + 1) / 2;
+        {
+            double bR[T][T], bI[T][T];
+            int a = 2;
+#pragma unroll 1
+            for (; a + 1 <= Ahalf; a += 2) {
+                sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                                         Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
+                qo = sym_tdot<T>(Pw, bR, bI, L, h);  // Pw holds K_(a-1): tr(B^(2a-1))
+                wave_lds_sync();
+                sym_store<T>(Pw, bR, bI, L, fm);
+                wave_lds_sync();
+                sym_product_epi<T, LOOP>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
+                                         Zb + (a & 1) * NP, tu, h, pt, lv);
+                qo = sym_tdot<T>(Pw, aR, aI, L, h);
+                wave_lds_sync();
+                sym_store<T>(Pw, aR, aI, L, fm);
+                wave_lds_sync();
+            }
+            if (a <= Ahalf) {  // one power left: K_a into b, then its epilogue alone
+                sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                                         Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
+                qo = sym_tdot<T>(Pw, bR, bI, L, h);
+                wave_lds_sync();
+                sym_store<T>(Pw, bR, bI, L, fm);
+                wave_lds_sync();
+                epis_alone<T, LOOP>(bR, bI, a, qo, L, fm, lane, Pw, Ms, Zb + ((a - 1) & 1) * NP, Zb + (a & 1) * NP,
+                                    tu, h, pt, lv);
+            } else {  // the last power (a - 1) is in a
+                epis_alone<T, LOOP>(aR, aI, a - 1, qo, L, fm, lane, Pw, Ms, Zb + (a & 1) * NP,
+                                    Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
+            }
+        }
+        cplx v1 = cmk(0.0, 0.0);
+        if (LOOP) {
+            // v_1 = XD_d . D = sum_e d_e D[swap e] D[e] (every lane holds the sum)
+            cplx part = cmk(0.0, 0.0);
+            for (int e = lane; e < n; e += WH_WAVE) cmac(part, cmulr(Dv[swp(e, h)], sgn(tu, e, h)), Dv[e]);
+            v1 = wave_csum(part);
+        }
+        wave_lds_sync();  // pt and lv as the epilogues stored them
+        // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2)
+        const cplx top = coeff_top_s<4 * T, LOOP>(pt, lv, v1, h);
+        if (lane == h) acc_sum.add(cmulr(top, prefac));
+    }
+
+    // fold the waves of this workgroup in wave order
+    if (lane == h) wres[wave] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        csum s;
+        s.zero();
+        for (int w = 0; w < nw; ++w) {
+            double4 pq = wres[w];
+            s.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        partials[b] = make_double4(s.sr, s.cr, s.si, s.ci);
+    }
+}
+
+}  // namespace
+
+template <int T, bool LOOP>
+__global__ void __launch_bounds__(256)
+wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
+              double4* __restrict__ partials, int64_t block_offset) {
+    extern __shared__ __align__(16) unsigned char wh_smem[];
+    const int64_t b = blockIdx.x + block_offset;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
+    }
+    pm1s_body<T, LOOP>(jobs[lo], pool, partials, b, wh_smem);
+}
+
+size_t wh_pm1s_lds_bytes(int T, int nw) {
+    size_t NP = 8 * (size_t)T, S = NP + WH_PM1S_PAD;
+    size_t mat = (NP * S * sizeof(cplx) + 31) / 32 * 32;
+    size_t small = (2 * (WH_EMAX + 2) + 2 * NP) * sizeof(cplx);
+    return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
+}
+
+template <int T, bool LOOP>
+static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
+                           int64_t block_offset, int nw, hipStream_t stream) {
+    size_t lds = wh_pm1s_lds_bytes(T, nw);
+    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1s<T, LOOP>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((wh_sieve_pm1s<T, LOOP>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream, jobs,
+                       njobs, (const cplx*)pool, (double4*)partials, block_offset);
+    return hipGetLastError();
+}
+
+hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                          int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+#define WH_PM1S_CASE(TT)                                                                                   \
+    case TT:                                                                                               \
+        return loop ? launch_s<TT, true>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream)  \
+                    : launch_s<TT, false>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    switch (T) {
+        WH_PM1S_CASE(1)
+        WH_PM1S_CASE(2)
+        WH_PM1S_CASE(3)
+        WH_PM1S_CASE(4)
+        WH_PM1S_CASE(5)
+        default: return hipErrorInvalidValue;
+    }
+#undef WH_PM1S_CASE
+}

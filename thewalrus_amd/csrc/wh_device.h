@@ -161,8 +161,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 __device__ __forceinline__ cplx wave_csum(cplx z) { return cmk(wave_sum(z.re), wave_sum(z.im)); }
 
-// Binomial coefficient as the reference's table holds it (_hafnian.py:27-45), in double.
-// Binomial coefficient, exact in 64-bit: each step divides out the common factor first, so no
+// Lanes of one wave exchanging data through LDS: the hardware completes a wave's LDS operations
+// in issue order, but the compiler reasons per lane and may move a load above another lane's
+// store (or forward a conditional store to its own lane only).  The wavefront-scope fences keep
+// every LDS access on its side of this point; they emit no instruction.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Binomial coefficient as the reference's table holds it (_hafnian.py:27-45), in double,
+// exact in 64-bit: each step divides out the common factor first, so no
 // intermediate exceeds the result (C(64, 32) < 2^64 covers every multiplicity up to WH_NMAX / 2).
 __device__ __forceinline__ double binom_d(int n, int k) {
     if (k < 0 || k > n) return 0.0;
