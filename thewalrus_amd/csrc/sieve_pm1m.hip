@@ -35,6 +35,10 @@ static __constant__ double kInvJm[WH_EMAX + 1] = {
 __device__ __forceinline__ double mfma4(double a, double b, double c) {
     return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
 }
+// c + a (-b): for the f64 instructions the last operand holds the negate bits (A, B, C)
+__device__ __forceinline__ double mfma4_nb(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 2);
+}
 
 // factor of column c of the current term (+1 digit set; -1 Glynn / 0 montrealer otherwise)
 __device__ __forceinline__ double col_factor(uint64_t t, int c, int h, double unset) {
@@ -147,12 +151,8 @@ __device__ __forceinline__ void mfma_product(const cplx* __restrict__ Pw, const 
         cplx a[T], bq[T];
 #pragma unroll
         for (int p = 0; p < T; ++p) a[p] = pa[8 * p * S + 4 * kc];
-        double nbi[T];
 #pragma unroll
-        for (int q = 0; q < T; ++q) {
-            bq[q] = cmulr(pb[4 * kc * S + 8 * q], sg[q]);
-            nbi[q] = -bq[q].im;
-        }
+        for (int q = 0; q < T; ++q) bq[q] = cmulr(pb[4 * kc * S + 8 * q], sg[q]);
 #pragma unroll
         for (int p = 0; p < T; ++p)
 #pragma unroll
@@ -164,7 +164,7 @@ __device__ __forceinline__ void mfma_product(const cplx* __restrict__ Pw, const 
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = 0; q < T; ++q) {
-                aR[p][q] = mfma4(a[p].im, nbi[q], aR[p][q]);
+                aR[p][q] = mfma4_nb(a[p].im, bq[q].im, aR[p][q]);
                 aI[p][q] = mfma4(a[p].im, bq[q].re, aI[p][q]);
             }
     }
@@ -366,14 +366,10 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
         cplx av[T], bq[T];
-        double nbi[T];
 #pragma unroll
         for (int p = 0; p < T; ++p) av[p] = avn[p];
 #pragma unroll
-        for (int q = 0; q < T; ++q) {
-            bq[q] = cmulr(bqn[q], sg[q]);
-            nbi[q] = -bq[q].im;
-        }
+        for (int q = 0; q < T; ++q) bq[q] = cmulr(bqn[q], sg[q]);
         if (kc + 1 < NK) {
 #pragma unroll
             for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S + 4 * (kc + 1)];
@@ -399,7 +395,7 @@ __device__ __forceinline__ void mfma_product_epi(const cplx* __restrict__ Pw, co
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = 0; q < T; ++q) {
-                nR[p][q] = mfma4(av[p].im, nbi[q], nR[p][q]);
+                nR[p][q] = mfma4_nb(av[p].im, bq[q].im, nR[p][q]);
                 nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
                 if (st < nst)
                     epi_step<T, LOOP, YS>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h,

@@ -5,22 +5,22 @@
 //
 // With A = Ax symmetric, X the swap of the two halves and D_d = diag(d) (d = +-1 repeating over
 // the halves, so X D_d = D_d X and (X D_d)^2 = I), the term matrix is B = A X D_d and
-//     C_a = B^a X D_d = A (X D_d A)^(a-1)   is symmetric.
-// The kernel keeps K_a = C_a D_d = B^a X (K_a[i][k] = C_a[i][k] d_k):
-//     K_1 = A D_d,   K_(a+1) = K_a (X A) D_d,
-// the product of sieve_pm1m.hip with X A for A X, formed for the upper block groups only
-// (T (T + 1) / 2 of the T^2 groups); the LDS copy is completed from the symmetry of C,
-// K[k][i] = d_i d_k K[i][k].  The power traces need no signs:
-//     tr(B^(a+b)) = sum_(i,k) d_i d_k C_a[i][k] C_b[swap k][swap i]
-//                 = sum_(i,k) K_a[i][k] K_b[swap k][swap i],
-// a summand symmetric in (i, k), so the upper groups off the diagonal count twice.  The loop
-// vectors z_a = B^a D follow the chain z_a = A (X D_d z_(a-1)), and the loop dots use
-// y_a = X D_d z_a as in sieve_pm1m.hip.  The montrealer (0/1 factors: D_d^2 is not I) stays on
-// sieve_pm1m.hip.
+//     C_a = B^a X D_d = A (X D_d A)^(a-1)   is symmetric, and so is V_a = D_d C_a D_d.
+// Each wave keeps V_a in LDS and forms the products
+//     R_(a+1) = V_a (X A) = D_d C_(a+1),   V_(a+1)[i][j] = R_(a+1)[i][j] d_j,
+// on the matrix cores for the upper block groups only (T (T + 1) / 2 of the T^2 groups of
+// sieve_pm1m.hip), with X A staged once per workgroup and no sign in the products: the lane
+// stores R d_j at (i, j) and, by symmetry, at (j, i).  The power traces are
+//     tr(B^(a+b)) = sum_(i,k) C_a[i][k] V_b[swap k][swap i] = sum_(i,k) d_i R_a[i][k] V_b[swap k][swap i],
+// a summand symmetric in (i, k): the upper groups off the diagonal count twice and the row sign
+// d_i scales each row group's partial sum once.  The loop vectors z_a = B^a D follow the chain
+// z_a = A (X D_d z_(a-1)), and the loop dots use y_a = X D_d z_a as in sieve_pm1m.hip.  The
+// montrealer (0/1 factors: D_d^2 is not I) stays on sieve_pm1m.hip.
 //
 // Lane map and MFMA layout as sieve_pm1m.hip: x = lane & 3, br = bit 2, bc = bit 3,
-// y = lane >> 4; group (p, q) holds K[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
-// rows and columns live in one mask register and flip the sign bit of the operands they scale.
+// y = lane >> 4; group (p, q) holds R[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
+// rows and columns live in one mask register and flip the sign bit of the values they scale;
+// the product's -Im(a) Im(b) term uses the matrix core's negate modifier on B.
 #include "wh_device.h"
 #include "wh_launch.h"
 
@@ -38,6 +38,10 @@ static __constant__ double kInvJs[WH_EMAX + 1] = {
 
 __device__ __forceinline__ double mfma4(double a, double b, double c) {
     return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+// c + a (-b): for the f64 instructions the last operand holds the negate bits (A, B, C)
+__device__ __forceinline__ double mfma4_nb(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 2);
 }
 
 // d_c of the current term: +1 if digit (c mod h) is set, -1 otherwise, 1 on the padding
@@ -107,25 +111,30 @@ __device__ __forceinline__ cplx coeff_top_s(const cplx* pt, const cplx* lv, cplx
     return res;
 }
 
-// lane part of sum_(i,k) Ka[i][k] Kb[swap k][swap i] over the upper groups (off-diagonal twice)
+// lane part of sum_(i,k) d_i Ra[i][k] Vb[swap k][swap i] over the upper groups (off-diagonal
+// twice), the row sign applied to each row group's partial sum
 template <int T>
-__device__ __forceinline__ cplx sym_tdot(const cplx* Kb, const double (&aR)[T][T], const double (&aI)[T][T],
-                                         const LaneMapS& L, int h) {
+__device__ __forceinline__ cplx sym_tdot(const cplx* Vb, const double (&aR)[T][T], const double (&aI)[T][T],
+                                         const LaneMapS& L, uint32_t fm, int h) {
     constexpr int S = GeoS<T>::S;
-    cplx sd = cmk(0.0, 0.0), so = cmk(0.0, 0.0);
+    cplx s = cmk(0.0, 0.0);
 #pragma unroll
-    for (int p = 0; p < T; ++p)
+    for (int p = 0; p < T; ++p) {
+        const int rs = swp(L.rd(p), h);
+        cplx sd = cmk(0.0, 0.0), so = cmk(0.0, 0.0);
 #pragma unroll
         for (int q = p; q < T; ++q) {
-            const cplx x = Kb[swp(L.cd(q), h) * S + swp(L.rd(p), h)];
+            const cplx x = Vb[swp(L.cd(q), h) * S + rs];
             if (q == p) cmac(sd, cmk(aR[p][q], aI[p][q]), x);
             else cmac(so, cmk(aR[p][q], aI[p][q]), x);
         }
-    return cadd(sd, cmulr(so, 2.0));
+        s = cadd(s, cflip(cadd(sd, cmulr(so, 2.0)), (fm >> p) & 1u));
+    }
+    return s;
 }
 
-// K into LDS: each upper group at its place and, off the diagonal, at the mirrored place with
-// the sign d_i d_k (bits p and T + q of the lane's sign mask)
+// V = R diag(d) into LDS: each upper group at its place and, off the diagonal, at the mirrored
+// place (V is symmetric); column sign = bit T + q of the lane's sign mask
 template <int T>
 __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], const double (&aI)[T][T],
                                           const LaneMapS& L, uint32_t fm) {
@@ -134,9 +143,9 @@ __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], con
     for (int p = 0; p < T; ++p)
 #pragma unroll
         for (int q = p; q < T; ++q) {
-            const cplx v = cmk(aR[p][q], aI[p][q]);
+            const cplx v = cflip(cmk(aR[p][q], aI[p][q]), (fm >> (T + q)) & 1u);
             X[L.rd(p) * S + L.cd(q)] = v;
-            if (p < q) X[L.cd(q) * S + L.rd(p)] = cflip(v, ((fm >> p) ^ (fm >> (T + q))) & 1u);
+            if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
         }
 }
 
@@ -146,7 +155,7 @@ __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], con
 // loop dots and sums the eight results over the wave.
 template <int T>
 struct EpiS {
-    cplx qe, qf, de, dodd, zs;
+    cplx qe, qf, qt, de, dodd, zs;
     cplx tx[T], av[T], tv[T], zc[3];
     double s1[4], s2[2], s3;
 };
@@ -158,14 +167,14 @@ __device__ __forceinline__ int epis_steps(int kc) {
     int n = 0;
     if ((kc & 1) == 0) n = (T - r) + (LOOP ? T : 0);
     else n = LOOP ? 4 : 0;
-    if (kc == NK - 1) n += (LOOP ? 2 : 0) + 12;
+    if (kc == NK - 1) n += (LOOP ? 2 : 0) + 11;
     return n;
 }
 
 template <int T, bool LOOP>
 __device__ __forceinline__ void epis_init(EpiS<T>& E, const LaneMapS& L, const cplx* zprev, uint32_t fm,
                                           int h) {
-    E.qe = E.qf = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
+    E.qe = E.qf = E.qt = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
     if (LOOP) {  // t = X D_d z_(a-1) at the lane's columns
 #pragma unroll
         for (int q = 0; q < T; ++q) E.tv[q] = cflip(zprev[swp(L.cd(q), h)], (fm >> (T + q)) & 1u);
@@ -174,12 +183,12 @@ __device__ __forceinline__ void epis_init(EpiS<T>& E, const LaneMapS& L, const c
 
 // the operands of row group r's micro-steps
 template <int T, bool LOOP>
-__device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS& L, const cplx* Kb,
+__device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS& L, const cplx* Vb,
                                               const cplx* Ms, int h) {
     constexpr int S = GeoS<T>::S;
     const int rs = swp(L.rd(r), h);
 #pragma unroll
-    for (int q = r; q < T; ++q) E.tx[q] = Kb[swp(L.cd(q), h) * S + rs];
+    for (int q = r; q < T; ++q) E.tx[q] = Vb[swp(L.cd(q), h) * S + rs];
     if (LOOP) {  // A[rd(r)][cd(q)] = (X A)[swap rd(r)][cd(q)]
 #pragma unroll
         for (int q = 0; q < T; ++q) E.av[q] = Ms[rs * S + L.cd(q)];
@@ -188,9 +197,9 @@ __device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS&
 
 template <int T, bool LOOP>
 __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const double (&oR)[T][T],
-                                          const double (&oI)[T][T], const LaneMapS& L, int lane, int a_prev,
-                                          cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt, cplx* lv,
-                                          uint64_t tu, int h) {
+                                          const double (&oI)[T][T], const LaneMapS& L, uint32_t fm, int lane,
+                                          int a_prev, cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt,
+                                          cplx* lv, uint64_t tu, int h) {
     constexpr int NK = 2 * T;
     const int r = kc >> 1;
     if ((kc & 1) == 0) {
@@ -198,6 +207,10 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
             const int q = r + st;
             if (q == r) cmac(E.qe, cmk(oR[r][q], oI[r][q]), E.tx[q]);
             else cmac(E.qf, cmk(oR[r][q], oI[r][q]), E.tx[q]);
+            if (q == T - 1) {  // the row group's partial sum, times its row sign
+                E.qt = cadd(E.qt, cflip(cadd(E.qe, cmulr(E.qf, 2.0)), (fm >> r) & 1u));
+                E.qe = E.qf = cmk(0.0, 0.0);
+            }
             return;
         }
         if (LOOP && st < T - r + T) {
@@ -234,9 +247,7 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
         }
         st -= 2;
     }
-    if (st == 0) { E.qe = cadd(E.qe, cmulr(E.qf, 2.0)); return; }
-    st -= 1;
-    const double v[8] = {qo_prev.re, qo_prev.im, E.qe.re, E.qe.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
+    const double v[8] = {qo_prev.re, qo_prev.im, E.qt.re, E.qt.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
     if (st < 4) { E.s1[st] = dswap32(v[st], v[st + 4]); return; }
     if (st < 6) { E.s2[st - 4] = dswap16(E.s1[st - 4], E.s1[st - 2]); return; }
     if (st == 6) {
@@ -261,11 +272,11 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
     }
 }
 
-// Kn = Kb (X A) D_d for the upper groups (fully unrolled k chunks) with power a_prev's epilogue
-// (from Ko and Kb = Pw = K_(a_prev)) between the MFMA pairs; operands of chunk kc + 1 load
+// Rn = Vb (X A) for the upper groups (fully unrolled k chunks) with power a_prev's epilogue
+// (from Ro and Vb = Pw = V_(a_prev)) between the MFMA pairs; operands of chunk kc + 1 load
 // while chunk kc runs.
 template <int T, bool LOOP>
-__device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Kb, const cplx* __restrict__ Ms,
+__device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, const cplx* __restrict__ Ms,
                                                 double (&nR)[T][T], double (&nI)[T][T], const LaneMapS& L,
                                                 uint32_t fm, const double (&oR)[T][T], const double (&oI)[T][T],
                                                 int a_prev, cplx qo_prev, int lane, const cplx* zprev,
@@ -281,32 +292,28 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Kb, con
         }
     EpiS<T> E;
     epis_init<T, LOOP>(E, L, zprev, fm, h);
-    const cplx* pa = Kb + (4 * L.br + L.x) * S + L.y;
+    const cplx* pa = Vb + (4 * L.br + L.x) * S + L.y;
     const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
     cplx avn[T], bqn[T];
 #pragma unroll
     for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S];
 #pragma unroll
     for (int q = 0; q < T; ++q) bqn[q] = pb[8 * q];
-    epis_load_row<T, LOOP>(0, E, L, Kb, Ms, h);
+    epis_load_row<T, LOOP>(0, E, L, Vb, Ms, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
         cplx av[T], bq[T];
-        double nbi[T];
 #pragma unroll
         for (int p = 0; p < T; ++p) av[p] = avn[p];
 #pragma unroll
-        for (int q = 0; q < T; ++q) {
-            bq[q] = cflip(bqn[q], (fm >> (T + q)) & 1u);  // column factor d_j
-            nbi[q] = -bq[q].im;
-        }
+        for (int q = 0; q < T; ++q) bq[q] = bqn[q];
         if (kc + 1 < NK) {
 #pragma unroll
             for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S + 4 * (kc + 1)];
 #pragma unroll
             for (int q = 0; q < T; ++q) bqn[q] = pb[4 * (kc + 1) * S + 8 * q];
         }
-        if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP>((kc + 1) >> 1, E, L, Kb, Ms, h);
+        if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP>((kc + 1) >> 1, E, L, Vb, Ms, h);
         const int nst = epis_steps<T, LOOP>(kc);
         int st = 0;
 #pragma unroll
@@ -315,7 +322,7 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Kb, con
             for (int q = p; q < T; ++q) {
                 nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
                 nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
-                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -323,21 +330,21 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Kb, con
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = p; q < T; ++q) {
-                nR[p][q] = mfma4(av[p].im, nbi[q], nR[p][q]);
+                nR[p][q] = mfma4_nb(av[p].im, bq[q].im, nR[p][q]);
                 nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
-                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
         for (; st < nst; ++st)
-            epis_step<T, LOOP>(kc, st, E, oR, oI, L, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+            epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
     }
 }
 
 // the last power's epilogue, the same micro-steps in sequence
 template <int T, bool LOOP>
 __device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const double (&aI)[T][T], int a, cplx qo,
-                                           const LaneMapS& L, uint32_t fm, int lane, const cplx* Kb,
+                                           const LaneMapS& L, uint32_t fm, int lane, const cplx* Vb,
                                            const cplx* Ms, const cplx* zprev, cplx* zcur, uint64_t tu, int h,
                                            cplx* pt, cplx* lv) {
     constexpr int NK = 2 * T;
@@ -345,11 +352,11 @@ __device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const doubl
     epis_init<T, LOOP>(E, L, zprev, fm, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
-        if ((kc & 1) == 0) epis_load_row<T, LOOP>(kc >> 1, E, L, Kb, Ms, h);
+        if ((kc & 1) == 0) epis_load_row<T, LOOP>(kc >> 1, E, L, Vb, Ms, h);
         const int nst = epis_steps<T, LOOP>(kc);
 #pragma unroll
         for (int st = 0; st < 2 * T + 18; ++st)
-            if (st < nst) epis_step<T, LOOP>(kc, st, E, aR, aI, L, lane, a, qo, zprev, zcur, pt, lv, tu, h);
+            if (st < nst) epis_step<T, LOOP>(kc, st, E, aR, aI, L, fm, lane, a, qo, zprev, zcur, pt, lv, tu, h);
     }
 }
 
@@ -369,7 +376,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     const int n = 2 * J.E;
     const int h = J.E;
 
-    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one K per wave, D, per wave pt, lv, z rows x 2
+    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave, D, per wave pt, lv, z rows x 2
     cplx* Ms = reinterpret_cast<cplx*>(smem);
     const size_t mat = ((size_t)NP * S * sizeof(cplx) + 31) / 32 * 32 / sizeof(cplx);
     cplx* Pw = Ms + mat + wave * mat;
@@ -412,13 +419,13 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
 #pragma unroll
         for (int q = 0; q < T; ++q) fm |= negbit(tu, L.cd(q), h) << (T + q);
 
-        // K_1 = A D_d (upper groups), into LDS whole
+        // R_1 = D_d A (upper groups); V_1 = D_d A D_d into LDS whole
         double aR[T][T], aI[T][T];
 #pragma unroll
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = p; q < T; ++q) {
-                const cplx m = cflip(Ms[swp(L.rd(p), h) * S + L.cd(q)], (fm >> (T + q)) & 1u);
+                const cplx m = cflip(Ms[swp(L.rd(p), h) * S + L.cd(q)], (fm >> p) & 1u);
                 aR[p][q] = m.re;
                 aI[p][q] = m.im;
             }
@@ -430,8 +437,8 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
             for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];  // z_0 = D
         }
         wave_lds_sync();
-        // powers up to K_A, A = ceil(h/2), software pipelined as sieve_pm1m.hip: the product
-        // forming K_a runs with the epilogue of K_(a-1); the accumulators alternate between two
+        // powers up to A = ceil(h/2), software pipelined as sieve_pm1m.hip: the product forming
+        // R_a runs with the epilogue of power a - 1; the accumulators alternate between two
         // register sets
         const int Ahalf = (h + 1) / 2;
         {
@@ -441,21 +448,21 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
             for (; a + 1 <= Ahalf; a += 2) {
                 sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, bR, bI, L, h);  // Pw holds K_(a-1): tr(B^(2a-1))
+                qo = sym_tdot<T>(Pw, bR, bI, L, fm, h);  // Pw holds V_(a-1): tr(B^(2a-1))
                 wave_lds_sync();
                 sym_store<T>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
                 sym_product_epi<T, LOOP>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
                                          Zb + (a & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, aR, aI, L, h);
+                qo = sym_tdot<T>(Pw, aR, aI, L, fm, h);
                 wave_lds_sync();
                 sym_store<T>(Pw, aR, aI, L, fm);
                 wave_lds_sync();
             }
-            if (a <= Ahalf) {  // one power left: K_a into b, then its epilogue alone
+            if (a <= Ahalf) {  // one power left: R_a into b, then its epilogue alone
                 sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, bR, bI, L, h);
+                qo = sym_tdot<T>(Pw, bR, bI, L, fm, h);
                 wave_lds_sync();
                 sym_store<T>(Pw, bR, bI, L, fm);
                 wave_lds_sync();

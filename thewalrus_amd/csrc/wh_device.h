@@ -107,11 +107,13 @@ struct csum {
 // Every step adds a lane's value to its partner's; partners add the same two numbers, so all
 // lanes of a group end with bit-identical sums.  Callers keep the whole wave active.
 
+// Every control used here has a source lane inside the row for every lane, so bound_ctrl (a
+// lane without a source reads 0) never applies and the move needs no initialised destination.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 #define WH_DPP_XOR1 0xB1        // quad_perm [1,0,3,2]
