@@ -191,10 +191,15 @@ struct GroupKey {
 // powers C_a = B^a X D_d (KIND_PM1S, upper block groups only), the montrealer (whose 0/1
 // factors break that symmetry) and WH_OPT_NONSYM requests with the full powers (KIND_PM1M).
 static void pick_matrix_cores(GroupKey& g, int options) {
-    // T = 6 (n in [42, 48]) keeps 72 complex accumulators per lane and spills on the matrix
-    // cores: it stays on the vector kernel
-    if (g.kind == KIND_PM1 && !g.real && g.half && g.T <= 5 && !(options & WH_OPT_VALU))
-        g.kind = (g.mtl || (options & WH_OPT_NONSYM)) ? KIND_PM1M : KIND_PM1S;
+    if (g.kind != KIND_PM1 || g.real || !g.half || (options & WH_OPT_VALU)) return;
+    const bool full = g.mtl || (options & WH_OPT_NONSYM);
+    // full powers at T = 6 (n in [42, 48]) keep 72 complex accumulators per lane and spill: the
+    // vector kernel keeps them; the symmetric kernel's 21 upper groups fit
+    if (full) {
+        if (g.T <= 5) g.kind = KIND_PM1M;
+    } else {
+        g.kind = KIND_PM1S;
+    }
 }
 
 GroupKey classify(const wh_job_dev& j, int options) {

@@ -547,6 +547,7 @@ hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, c
         WH_PM1S_CASE(3)
         WH_PM1S_CASE(4)
         WH_PM1S_CASE(5)
+        WH_PM1S_CASE(6)
         default: return hipErrorInvalidValue;
     }
 #undef WH_PM1S_CASE
