@@ -331,9 +331,15 @@ def main():
         sb = B.DeviceBatch([B.prepare(A, loop=l) for A, l in sm])
         sb.run(stream)
         _, w = timed_runs(sb, 3, torch, stream)
+        r_eager = sb.results()
+        sb.capture(stream)  # the same plan as one HIP graph launch
+        sb.run(stream)
+        _, wg = timed_runs(sb, 3, torch, stream)
         rate, _ = per_call(sm, tw)
         small = {"requests": len(sm), "batched": 3 * len(sm) / w, "per_call": rate,
-                 "batched_over_per_call": 3 * len(sm) / w / rate}
+                 "batched_over_per_call": 3 * len(sm) / w / rate,
+                 "batched_hip_graph": 3 * len(sm) / wg,
+                 "graph_equals_eager": bool(np.array_equal(sb.results(), r_eager))}
         sb.close()
 
     configs = None if args.no_configs else configs_run(torch, tw, B, stream)

@@ -185,6 +185,32 @@ int wh_plan_chunks(const wh_plan* plan, int64_t* first_chunk, int64_t* nchunks, 
 
 int wh_plan_destroy(wh_plan* plan);
 
+/* ---- HIP graphs.  No counterpart in the reference (numba launches its prange loop per call);
+ * for callers that replay one plan many times, such as a serving loop over a fixed batch. */
+typedef struct wh_graph wh_graph;
+
+/* Captures wh_plan_run(plan, dpool, dout) on `stream` (a hipStream_t other than the default
+ * stream) into an instantiated HIP graph.  Nothing runs during capture; the plan, dpool and
+ * dout must outlive the graph.  The captured launches hold no memset nodes. */
+int wh_plan_graph_create(wh_plan* plan, const double* dpool, double* dout, void* stream, wh_graph** graph);
+
+/* Replays the captured launches on `stream`; no allocation, no host synchronisation. */
+int wh_graph_launch(wh_graph* graph, void* stream);
+
+int wh_graph_destroy(wh_graph* graph);
+
+/* ---- Term-chunk tuning.  No counterpart in the reference (numba's prange picks its own
+ * chunks); set by the autotuner (thewalrus_amd/tune.py).  The chunk size of a request aims at
+ * `blocks_per_request` workgroups (0 = the default: 1024 for the all-ones kernels, 256 for the
+ * general kernel), per kernel class: kind 0 = all-ones kernels with tile side class T = 1..6
+ * (n_pad = 8T), kind 1 = general kernel with T = 0; loop = 0 or 1.  Plans and batches created
+ * afterwards use the table; a request's result depends on its chunking only in the last bits
+ * (summation order). */
+int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
+
+/* The table entry, or -1 for an entry out of range. */
+int64_t wh_tune_get(int kind, int T, int loop);
+
 #ifdef __cplusplus
 }
 #endif

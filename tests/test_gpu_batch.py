@@ -163,3 +163,81 @@ def test_matrix_cores_and_vector_units(loop):
             v = res["vector units"][k]
             for name in ("symmetric powers", "full powers"):
                 assert relerr(res[name][k], v) < 1e-10, (name, n, relerr(res[name][k], v))
+
+
+def test_graph_replay_equals_eager():
+    """A plan captured into a HIP graph replays bit-identically to the eager launch, on every
+    replay, with other device work (another plan, torch kernels) between the replays."""
+    import torch
+
+    rng = np.random.default_rng(71)
+    sizes = [6, 10, 16, 20, 24, 30, 8, 12]
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), i % 2 == 1) for i, n in enumerate(sizes)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    other = B.DeviceBatch([B.prepare(random_symmetric(rng, 14), loop=True)])
+    eager = B.DeviceBatch(preps)
+    eager.run()
+    ref = eager.results()
+    eager.close()
+    s = torch.cuda.Stream()
+    g = B.DeviceBatch(preps)
+    g.capture(s)
+    for rep in range(4):
+        g.out.zero_()
+        torch.cuda.synchronize()
+        g.run(s)
+        s.synchronize()
+        got = g.results()
+        assert np.array_equal(got, ref), rep
+        other.run()  # eager work between replays
+        x = torch.ones(1 << 16, dtype=torch.float64, device="cuda").sum()
+        torch.cuda.synchronize()
+        assert float(x) == 65536.0
+    g.close()
+    other.close()
+
+
+def test_graph_needs_a_stream():
+    import torch
+
+    rng = np.random.default_rng(72)
+    g = B.DeviceBatch([B.prepare(random_symmetric(rng, 8))])
+    with pytest.raises(ValueError):
+        g.capture(torch.cuda.default_stream())
+    g.close()
+
+
+def test_tuned_chunking_changes_chunks_not_results():
+    """Another workgroups-per-request target changes the chunking of that class only (all-ones
+    kernel, T = 4, loop); the results move in their last bits at most."""
+    import ctypes
+
+    from thewalrus_amd import _native, tune
+
+    rng = np.random.default_rng(73)
+    reqs = [(random_symmetric(rng, 32, scale=1 / np.sqrt(32)), True), (random_symmetric(rng, 28), False)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    before = tune.current()
+
+    def chunks(db):
+        ct = (ctypes.c_int64 * db.nreq)()
+        _native.check(_native.load().wh_plan_chunks(db.plan, None, None, ct))
+        return list(ct)
+
+    try:
+        tune.apply([{"kind": 0, "T": 4, "loop": 1, "blocks_per_request": 512}])
+        a = B.DeviceBatch(preps)
+        a.run()
+        ra, ca = a.results(), chunks(a)
+        a.close()
+        tune.apply([{"kind": 0, "T": 4, "loop": 1, "blocks_per_request": 2048}])
+        b = B.DeviceBatch(preps)
+        b.run()
+        rb, cb = b.results(), chunks(b)
+        b.close()
+    finally:
+        tune.apply([{"kind": k, "T": T, "loop": lp, "blocks_per_request": v} for (k, T, lp), v in before.items()])
+    print("chunk terms", ca, cb, flush=True)
+    assert cb[0] < ca[0] and cb[1] == ca[1]
+    assert relerr(ra[0], rb[0]) < 1e-12
+    assert ra[1] == rb[1]

@@ -54,6 +54,11 @@ EXPORTS = [
     "wh_plan_partials",
     "wh_plan_chunks",
     "wh_plan_destroy",
+    "wh_plan_graph_create",
+    "wh_graph_launch",
+    "wh_graph_destroy",
+    "wh_tune_set",
+    "wh_tune_get",
     "wh_loop_hafnian_batch",
     "wh_loop_hafnian_batch_gamma",
     "wh_montrealer",
@@ -101,6 +106,13 @@ def load(path=None):
         L.wh_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.wh_plan_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
         L.wh_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.wh_plan_graph_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_void_p)]
+        L.wh_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_graph_destroy.argtypes = [ctypes.c_void_p]
+        L.wh_tune_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64]
+        L.wh_tune_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.wh_tune_get.restype = ctypes.c_int64
         L.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_void_p)]
         L.wh_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _ip, _ip, _dp]
@@ -121,6 +133,10 @@ def load(path=None):
                                                   ctypes.c_int64]
         for name in EXPORTS:
             getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
+        # the autotuner's chunk table (tuning.json next to the package, or $WH_TUNING)
+        from . import tune
+
+        tune.load(lib=L)
         if path is None:
             _lib = L
         return L

@@ -133,6 +133,7 @@ class DeviceBatch:
             jobs.append(p.job)
         self.nreq = len(jobs)
         self.plan = None
+        self.graph = None
         if self.nreq == 0:
             return
         options = ((1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
@@ -153,14 +154,32 @@ class DeviceBatch:
         self.terms, self.blocks, self.flops = terms.value, blocks.value, flops.value
 
     def run(self, stream=None):
-        """Enqueue the plan on ``stream`` (torch stream; default: the current stream)."""
+        """Enqueue the plan on ``stream`` (torch stream; default: the current stream), as one
+        graph launch after ``capture``."""
         if self.plan is None:
             return
         torch = self.torch
         with torch.cuda.device(self.device):
             s = stream if stream is not None else torch.cuda.current_stream(self.device)
+            if self.graph is not None:
+                _native.check(_native.load().wh_graph_launch(self.graph, ctypes.c_void_p(s.cuda_stream)))
+                return
             _native.check(_native.load().wh_plan_run(self.plan, self.pool.data_ptr(), self.out.data_ptr(),
                                                      ctypes.c_void_p(s.cuda_stream)))
+
+    def capture(self, stream):
+        """Capture the plan's launches on ``stream`` (a torch stream other than the default
+        stream) into a HIP graph; later ``run`` calls replay it with one launch.  Nothing runs
+        during capture."""
+        if self.plan is None or self.graph is not None:
+            return
+        if stream is None or stream.cuda_stream == 0:
+            raise ValueError("graph capture needs a stream other than the default stream")
+        with self.torch.cuda.device(self.device):
+            g = ctypes.c_void_p()
+            _native.check(_native.load().wh_plan_graph_create(self.plan, self.pool.data_ptr(), self.out.data_ptr(),
+                                                              ctypes.c_void_p(stream.cuda_stream), ctypes.byref(g)))
+            self.graph = g
 
     def results(self):
         """Synchronise and return all results (numpy complex128, request order)."""
@@ -174,6 +193,9 @@ class DeviceBatch:
         return out
 
     def close(self):
+        if self.graph is not None:
+            _native.load().wh_graph_destroy(self.graph)
+            self.graph = None
         if self.plan is not None:
             _native.load().wh_plan_destroy(self.plan)
             self.plan = None

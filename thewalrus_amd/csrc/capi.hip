@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -247,16 +248,27 @@ int pm1_waves(int T, bool real, int kind) {
     return best;
 }
 
+// Workgroups per request the chunk size aims at, per kernel class (wh_tune_set; 0 = the
+// defaults below): [0][T][loop] all-ones kernels by tile side class, [1][0][loop] general kernel.
+std::atomic<int64_t> g_tune_blocks[2][kPm1MaxT + 1][2];
+
+int64_t tuned_blocks(const GroupKey& g) {
+    const int kind = all_ones_kind(g.kind) ? 0 : 1;
+    const int T = kind == 0 ? std::min(std::max(g.T, 0), kPm1MaxT) : 0;
+    return g_tune_blocks[kind][T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
+}
+
 int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
+    const int64_t tb = tuned_blocks(g);
     if (all_ones_kind(g.kind)) {
-        int64_t c = j.steps / 1024;
+        int64_t c = j.steps / (tb > 0 ? tb : 1024);
         if (c < 4 * nw) c = 4 * nw;
         if (c > 4096) c = 4096;
         c = (c + nw - 1) / nw * nw;
         return c;
     }
     // generic: one wave per workgroup walks the chunk
-    int64_t c = j.steps / 256;
+    int64_t c = j.steps / (tb > 0 ? tb : 256);
     if (c < 1) c = 1;
     if (c > 256) c = 256;
     return c;
@@ -377,6 +389,12 @@ struct wh_plan {
     Layout layout;
     wh_job_dev* d_jobs = nullptr;
     double4* d_partials = nullptr;
+};
+
+struct wh_graph {
+    int device = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 extern "C" {
@@ -518,6 +536,63 @@ int wh_plan_destroy(wh_plan* p) {
     if (p->d_partials) (void)hipFree(p->d_partials);
     delete p;
     return WH_OK;
+}
+
+int wh_plan_graph_create(wh_plan* p, const double* dpool, double* dout, void* stream, wh_graph** out) {
+    if (!p || !dpool || !dout || !out) return fail(WH_EINVAL, "null plan, buffer or graph");
+    if (!stream) return fail(WH_EINVAL, "graph capture needs a stream other than the default stream");
+    hipStream_t s = (hipStream_t)stream;
+    wh_graph* g = new wh_graph();
+    g->device = p->device;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(WH_EDEVICE, "stream capture failed to start: %s", hipGetErrorString(e));
+    }
+    hipError_t er = run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s);
+    hipError_t ee = hipStreamEndCapture(s, &g->graph);
+    if (er != hipSuccess || ee != hipSuccess) {
+        wh_graph_destroy(g);
+        return fail(WH_EDEVICE, "plan capture failed: %s",
+                    hipGetErrorString(er != hipSuccess ? er : ee));
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        wh_graph_destroy(g);
+        return fail(WH_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return WH_OK;
+}
+
+int wh_graph_launch(wh_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(WH_EINVAL, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return WH_OK;
+}
+
+int wh_graph_destroy(wh_graph* g) {
+    if (!g) return WH_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return WH_OK;
+}
+
+int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request) {
+    if (kind < 0 || kind > 1 || loop < 0 || loop > 1 || blocks_per_request < 0 ||
+        (kind == 0 && (T < 1 || T > kPm1MaxT)) || (kind == 1 && T != 0))
+        return fail(WH_EINVAL, "tune entry out of range (kind %d, T %d, loop %d, blocks %lld)", kind, T, loop,
+                    (long long)blocks_per_request);
+    g_tune_blocks[kind][T][loop].store(blocks_per_request, std::memory_order_relaxed);
+    return WH_OK;
+}
+
+int64_t wh_tune_get(int kind, int T, int loop) {
+    if (kind < 0 || kind > 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
+        (kind == 1 && T != 0))
+        return -1;
+    return g_tune_blocks[kind][T][loop].load(std::memory_order_relaxed);
 }
 
 // Stages one request ([job | A | D | V] in one pinned block, one H2D copy), runs it and

@@ -1,0 +1,162 @@
+"""Autotuner for the term-chunk sizes of the sieve kernels.
+
+A request's 2^(h-1) (or prod(e_i + 1)) terms are cut into chunks, one workgroup per chunk.  The
+chunk size trades tail balance and latency-hiding across workgroups (many small chunks) against
+the per-workgroup cost of staging the matrix and folding partial sums (few large chunks).  The
+library aims at a number of workgroups per request per kernel class (`wh_tune_set`); this
+module measures candidate values on the device and keeps, per class, the fastest when it beats
+the built-in default by more than 1% (`choose`):
+
+* kind 0: the all-ones kernels (every multiplicity 1, even N), by tile side class T = 1..6
+  (n_pad = 8T), plain and loop;
+* kind 1: the general kernel (repeated rows, odd N, inclusion-exclusion), plain and loop.
+
+The table lives in a JSON file (`tuning.json` next to this module by default, or the path in
+the environment variable ``WH_TUNING``; ``WH_TUNING=none`` keeps the built-in defaults) that the
+library applies when it is loaded.  Tuning changes only the chunking: each result changes in
+its last bits at most (summation order), and stays deterministic for a given table.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _native
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PATH = os.path.join(HERE, "tuning.json")
+CANDIDATES = (256, 512, 1024, 2048, 4096)
+
+
+def table_path():
+    """The tuning file the library applies, or None when tuning is switched off."""
+    p = os.environ.get("WH_TUNING")
+    if p is None:
+        return DEFAULT_PATH
+    if p.strip().lower() in ("", "none", "off", "0"):
+        return None
+    return p
+
+
+def apply(entries, lib=None):
+    """Sets the library's table from a list of {"kind", "T", "loop", "blocks_per_request"}."""
+    L = lib or _native.load()
+    for e in entries:
+        rc = L.wh_tune_set(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e["blocks_per_request"]))
+        if rc != 0:
+            raise ValueError(L.wh_last_error().decode())
+
+
+def current(lib=None):
+    """The library's table: {(kind, T, loop): blocks_per_request}, 0 = built-in default."""
+    L = lib or _native.load()
+    out = {}
+    for kind, Ts in ((0, range(1, 7)), (1, (0,))):
+        for T in Ts:
+            for loop in (0, 1):
+                out[(kind, T, loop)] = int(L.wh_tune_get(kind, T, loop))
+    return out
+
+
+def load(path=None, lib=None):
+    """Applies a tuning file (default: ``table_path()``); returns its entries, or [] if none."""
+    p = path if path is not None else table_path()
+    if p is None or not os.path.exists(p):
+        return []
+    with open(p, encoding="utf-8") as f:
+        doc = json.load(f)
+    entries = doc.get("entries", [])
+    apply(entries, lib)
+    return entries
+
+
+def _class_workload(kind, T, loop, rng):
+    """Representative requests of one kernel class: about 2^18 terms in total (at most 256
+    requests), complex CN(0, 1/n) symmetric matrices."""
+    from . import batch as B
+
+    def sym(n):
+        X = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(2 * n)
+        return (X + X.T) / 2
+
+    if kind == 0:
+        n = 8 * T if T < 6 else 42  # n = 48 runs seconds per call; 42 is in the same class
+        h = n // 2
+        count = int(min(256, max(1, 2 ** 18 // 2 ** (h - 1))))
+        return [B.prepare(sym(n), loop=bool(loop)) for _ in range(count)]
+    # general kernel: repeated rows, 8 modes, 32 photons (the repeated-row request shape)
+    preps = []
+    for _ in range(256):
+        rpt = rng.multinomial(32, [1 / 8] * 8)
+        mu = (rng.normal(size=8) + 1j * rng.normal(size=8)) if loop else None
+        preps.append(B.prepare_repeated(sym(8), rpt, mu=mu, loop=bool(loop)))
+    return preps
+
+
+def _time(preps, reps, torch, stream):
+    from . import batch as B
+
+    db = B.DeviceBatch(preps)
+    try:
+        db.run(stream)  # warm-up (kernel attributes, caches)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            db.run(stream)
+            stream.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+    finally:
+        db.close()
+
+
+DEFAULT_BLOCKS = {0: 1024, 1: 256}  # the library's built-in targets (all-ones, general)
+
+
+def choose(kind, times, margin=0.01):
+    """The candidate to keep: the fastest, if it beats the built-in default by more than
+    ``margin`` (relative), else the default (timing noise does not move the table)."""
+    best = min(times, key=times.get)
+    d = DEFAULT_BLOCKS[kind]
+    if d in times and times[best] >= (1.0 - margin) * times[d]:
+        return d
+    return best
+
+
+def autotune(path=DEFAULT_PATH, candidates=CANDIDATES, reps=3, classes=None, seed=1234, log=print):
+    """Times every candidate per kernel class on the current device, applies the choice
+    (``choose``) and writes the table to ``path`` (None: do not write).  Returns the entries."""
+    import torch
+
+    L = _native.load()
+    rng = np.random.default_rng(seed)
+    if classes is None:
+        classes = [(0, T, loop) for T in range(1, 7) for loop in (0, 1)] + [(1, 0, 0), (1, 0, 1)]
+    stream = torch.cuda.Stream()
+    entries = []
+    for kind, T, loop in classes:
+        preps = _class_workload(kind, T, loop, rng)
+        times = {}
+        for c in candidates:
+            _native.check(L.wh_tune_set(kind, T, loop, int(c)))
+            times[c] = _time(preps, reps, torch, stream)
+        best = choose(kind, times)
+        _native.check(L.wh_tune_set(kind, T, loop, int(best)))
+        e = {"kind": kind, "T": T, "loop": loop, "blocks_per_request": int(best),
+             "ms": {str(c): round(1e3 * v, 4) for c, v in times.items()}}
+        entries.append(e)
+        if log:
+            log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s" % (kind, T, loop, best, e["ms"]))
+    if path is not None:
+        doc = {"device": torch.cuda.get_device_name(), "candidates": list(candidates), "entries": entries}
+        with open(path, "w", encoding="utf-8") as f:
+            json.dump(doc, f, indent=1)
+    return entries
+
+
+if __name__ == "__main__":
+    import sys
+
+    autotune(path=sys.argv[1] if len(sys.argv) > 1 else DEFAULT_PATH)
