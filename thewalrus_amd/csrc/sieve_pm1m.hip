@@ -1,6 +1,10 @@
-// sieve_pm1m.hip -- the all-ones Glynn sieve (and the montrealer) with the matrix products on
-// the FP64 matrix cores: the terms of _hafnian.py:443-462 (plain) and :546-569 (even loop)
-// when rpt is all ones, complex matrices, half-products formulation.
+// sieve_pm1m.hip -- the all-ones Glynn sieve with full powers on the FP64 matrix cores: the
+// terms of _hafnian.py:443-462 (plain) and :546-569 (even loop) when rpt is all ones, complex
+// matrices, half-products formulation, and the montrealer's subset terms
+// (_montrealer.py:56, :101).  Complex hafnians run on the symmetric-power kernel
+// (sieve_pm1s.hip, 60% of this kernel's matrix-core work at n_pad = 40) unless the request asks
+// for full powers (WH_OPT_NONSYM); the montrealer's 0/1 column factors break that kernel's
+// symmetry, so it stays here.
 //
 // Same work decomposition as sieve_pm1.hip (a workgroup keeps M = Ax X in LDS, each wave walks
 // terms and keeps its current power B^a in LDS); the product B^(a+1) = (B^a M) diag(d) runs as
