@@ -78,6 +78,12 @@ __device__ __forceinline__ double dswap16(double a, double b) {
 template <int T> struct GeoS {
     static constexpr int NP = 8 * T;
     static constexpr int S = NP + WH_PM1S_PAD;
+    // per term slot: p_1 .. p_(4T) and v_1 .. v_(4T+1) (with h <= 4T, a <= ceil(h/2)), the last
+    // entry the spare that lanes without a value write
+    static constexpr int PTN = 4 * T + 4;
+    // terms whose coefficients are formed together, one per lane (LDS budget at 4 waves, T <= 5;
+    // T = 6 runs 3 waves with the LDS nearly full)
+    static constexpr int NB = T == 6 ? 1 : T == 5 ? 7 : 8;
 };
 
 struct LaneMapS {
@@ -112,23 +118,39 @@ __device__ __forceinline__ cplx coeff_top_s(const cplx* pt, const cplx* lv, cplx
 }
 
 // lane part of sum_(i,k) d_i Ra[i][k] Vb[swap k][swap i] over the upper groups (off-diagonal
-// twice), the row sign applied to each row group's partial sum
+// twice), the row sign applied to each row group's partial sum.  Split in two so that the V
+// loads can be issued before the next power's store and summed after it (their latency then
+// overlaps the store).
 template <int T>
-__device__ __forceinline__ cplx sym_tdot(const cplx* Vb, const double (&aR)[T][T], const double (&aI)[T][T],
-                                         const LaneMapS& L, uint32_t fm, int h) {
+struct TdotVals {
+    cplx x[T * (T + 1) / 2];
+};
+template <int T>
+__device__ __forceinline__ void sym_tdot_load(TdotVals<T>& X, const cplx* Vb, const LaneMapS& L, int h) {
     constexpr int S = GeoS<T>::S;
-    cplx s = cmk(0.0, 0.0);
+    int g = 0;
 #pragma unroll
     for (int p = 0; p < T; ++p) {
         const int rs = swp(L.rd(p), h);
+#pragma unroll
+        for (int q = p; q < T; ++q) X.x[g++] = Vb[swp(L.cd(q), h) * S + rs];
+    }
+}
+template <int T>
+__device__ __forceinline__ cplx sym_tdot_sum(const TdotVals<T>& X, const double (&aR)[T][T],
+                                             const double (&aI)[T][T], uint32_t fm) {
+    cplx s = cmk(0.0, 0.0);
+    int g = 0;
+#pragma unroll
+    for (int p = 0; p < T; ++p) {
         cplx sd = cmk(0.0, 0.0), so = cmk(0.0, 0.0);
 #pragma unroll
-        for (int q = p; q < T; ++q) {
-            const cplx x = Vb[swp(L.cd(q), h) * S + rs];
-            if (q == p) cmac(sd, cmk(aR[p][q], aI[p][q]), x);
-            else cmac(so, cmk(aR[p][q], aI[p][q]), x);
+        for (int q = p; q < T; ++q, ++g) {
+            if (q == p) cmac(sd, cmk(aR[p][q], aI[p][q]), X.x[g]);
+            else cmac(so, cmk(aR[p][q], aI[p][q]), X.x[g]);
         }
-        s = cadd(s, cflip(cadd(sd, cmulr(so, 2.0)), (fm >> p) & 1u));
+        const cplx part = cflip(cadd(sd, cmulr(so, 2.0)), (fm >> p) & 1u);
+        s = p == 0 ? part : cadd(s, part);
     }
     return s;
 }
@@ -261,13 +283,13 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
     if (st == 9) { E.s3 += dpp_d<WH_DPP_HALF_MIRROR>(E.s3); return; }
     if (st == 10) {
         // lane 8k holds value k: p_ko, p_ke, v_2a, v_(2a+1) (re, im); the other lanes write the
-        // spare slot pt[WH_EMAX + 1] (never read)
+        // spare entry pt[PTN - 1] (never read)
         const int k = lane >> 3;
         const int a = a_prev;
         const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
         double* dst = k < 2 ? &pt[ko].re : k < 4 ? &pt[ke].re : k < 6 ? &lv[2 * a].re : &lv[2 * a + 1].re;
         dst += k & 1;
-        if ((lane & 7) != 0) dst = &pt[WH_EMAX + 1].re;
+        if ((lane & 7) != 0) dst = &pt[GeoS<T>::PTN - 1].re;
         *dst = E.s3;
     }
 }
@@ -376,16 +398,17 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     const int n = 2 * J.E;
     const int h = J.E;
 
-    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave, D, per wave pt, lv, z rows x 2
+    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave, D, per wave NB term slots (pt, lv)
+    // and z rows x 2
+    constexpr int PTN = GeoS<T>::PTN;
+    constexpr int NB = GeoS<T>::NB;
     cplx* Ms = reinterpret_cast<cplx*>(smem);
     const size_t mat = ((size_t)NP * S * sizeof(cplx) + 31) / 32 * 32 / sizeof(cplx);
     cplx* Pw = Ms + mat + wave * mat;
     cplx* Dv = Ms + (1 + nw) * mat;
-    constexpr int SMALL = 2 * (WH_EMAX + 2) + 2 * NP;
-    cplx* small = Dv + NP + wave * SMALL;
-    cplx* pt = small;
-    cplx* lv = pt + (WH_EMAX + 2);
-    cplx* Zb = lv + (WH_EMAX + 2);  // z_a by row, two buffers (a even / odd)
+    constexpr int SMALL = NB * 2 * PTN + 2 * NP;
+    cplx* slots = Dv + NP + wave * SMALL;
+    cplx* Zb = slots + NB * 2 * PTN;  // z_a by row, two buffers (a even / odd)
     double4* wres = reinterpret_cast<double4*>(Dv + NP + nw * SMALL);
 
     // ---- stage X A (row r holds A's row swap(r)), zero padded, and D
@@ -409,9 +432,14 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     csum acc_sum;
     acc_sum.zero();
 
+    // The coefficients of up to NB terms are formed together after their powers, lane j taking
+    // the j-th term of the batch (the exponential recurrence is the same work for one term or 64).
+    int slot = 0;
+    int64_t t_batch = t0 + wave;  // the batch's first term
     for (int64_t t = t0 + wave; t < t1; t += nw) {
         const uint64_t tu = (uint64_t)t;
-        const double prefac = ((h - __popcll(tu)) & 1) ? -1.0 : 1.0;
+        cplx* pt = slots + slot * 2 * PTN;
+        cplx* lv = pt + PTN;
         // the lane's signs: bit p for row rd(p), bit T + q for column cd(q)
         uint32_t fm = 0;
 #pragma unroll
@@ -443,29 +471,33 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
         const int Ahalf = (h + 1) / 2;
         {
             double bR[T][T], bI[T][T];
+            TdotVals<T> tv;
             int a = 2;
 #pragma unroll 1
             for (; a + 1 <= Ahalf; a += 2) {
                 sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, bR, bI, L, fm, h);  // Pw holds V_(a-1): tr(B^(2a-1))
+                sym_tdot_load<T>(tv, Pw, L, h);  // Pw holds V_(a-1): for tr(B^(2a-1))
                 wave_lds_sync();
                 sym_store<T>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
+                qo = sym_tdot_sum<T>(tv, bR, bI, fm);
                 sym_product_epi<T, LOOP>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
                                          Zb + (a & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, aR, aI, L, fm, h);
+                sym_tdot_load<T>(tv, Pw, L, h);
                 wave_lds_sync();
                 sym_store<T>(Pw, aR, aI, L, fm);
                 wave_lds_sync();
+                qo = sym_tdot_sum<T>(tv, aR, aI, fm);
             }
             if (a <= Ahalf) {  // one power left: R_a into b, then its epilogue alone
                 sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                qo = sym_tdot<T>(Pw, bR, bI, L, fm, h);
+                sym_tdot_load<T>(tv, Pw, L, h);
                 wave_lds_sync();
                 sym_store<T>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
+                qo = sym_tdot_sum<T>(tv, bR, bI, fm);
                 epis_alone<T, LOOP>(bR, bI, a, qo, L, fm, lane, Pw, Ms, Zb + ((a - 1) & 1) * NP, Zb + (a & 1) * NP,
                                     tu, h, pt, lv);
             } else {  // the last power (a - 1) is in a
@@ -473,21 +505,44 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
                                     Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
             }
         }
-        cplx v1 = cmk(0.0, 0.0);
         if (LOOP) {
             // v_1 = XD_d . D = sum_e d_e D[swap e] D[e] (every lane holds the sum)
             cplx part = cmk(0.0, 0.0);
             for (int e = lane; e < n; e += WH_WAVE) cmac(part, cmulr(Dv[swp(e, h)], sgn(tu, e, h)), Dv[e]);
-            v1 = wave_csum(part);
+            const cplx v1 = wave_csum(part);
+            if (lane == 0) lv[1] = v1;
         }
-        wave_lds_sync();  // pt and lv as the epilogues stored them
-        // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2)
-        const cplx top = coeff_top_s<4 * T, LOOP>(pt, lv, v1, h);
-        if (lane == h) acc_sum.add(cmulr(top, prefac));
+        ++slot;
+        if (slot == NB || t + nw >= t1) {
+            wave_lds_sync();  // pt and lv of every slot as the epilogues stored them
+            if (lane < slot) {
+                // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2)
+                const cplx* ptj = slots + lane * 2 * PTN;
+                const uint64_t tj = (uint64_t)(t_batch + (int64_t)lane * nw);
+                const double prefac = ((h - __popcll(tj)) & 1) ? -1.0 : 1.0;
+                const cplx top = coeff_top_s<4 * T, LOOP>(ptj, ptj + PTN, ptj[PTN + 1], h);
+                acc_sum.add(cmulr(top, prefac));
+            }
+            wave_lds_sync();  // the slots are free again
+            slot = 0;
+            t_batch = t + nw;
+        }
     }
 
-    // fold the waves of this workgroup in wave order
-    if (lane == h) wres[wave] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    // lanes 0 .. NB-1 hold the partial sums of their terms: fold them in lane order, then the
+    // waves of this workgroup in wave order
+    double4* lsum = reinterpret_cast<double4*>(slots);
+    if (lane < NB) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    wave_lds_sync();
+    if (lane == 0) {
+        csum s;
+        s.zero();
+        for (int j = 0; j < NB; ++j) {
+            double4 pq = lsum[j];
+            s.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        wres[wave] = make_double4(s.sr, s.cr, s.si, s.ci);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         csum s;
@@ -518,8 +573,9 @@ wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
 
 size_t wh_pm1s_lds_bytes(int T, int nw) {
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1S_PAD;
+    size_t PTN = 4 * (size_t)T + 4, NB = T == 6 ? 1 : T == 5 ? 7 : 8;
     size_t mat = (NP * S * sizeof(cplx) + 31) / 32 * 32;
-    size_t small = (2 * (WH_EMAX + 2) + 2 * NP) * sizeof(cplx);
+    size_t small = (NB * 2 * PTN + 2 * NP) * sizeof(cplx);
     return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
 
