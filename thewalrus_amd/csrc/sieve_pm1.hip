@@ -103,6 +103,12 @@ template <int T> struct Lds {
     static constexpr int S = 8 * T + WH_PM1_PAD;
 };
 
+// Per wave, NB term slots of p_0 .. p_(4T+1) and v_0 .. v_(4T+1) (h <= 4T; a spare last entry):
+// the coefficients of up to NB terms are formed together, one term per lane.  NB fits the LDS
+// beside the powers (complex T = 5 at 4 waves, T = 6 at 3 waves).
+__host__ __device__ constexpr int pm1_ptn(int T) { return 4 * T + 4; }
+__host__ __device__ constexpr int pm1_nb(int T, bool real) { return real ? 8 : T == 6 ? 2 : T == 5 ? 7 : 8; }
+
 template <int T, class E>
 __device__ __forceinline__ void tile_store(E* Pw, const E (&acc)[T][T], int I0, int J0) {
     wave_lds_sync();  // the other lanes' reads of the previous power are done
@@ -267,13 +273,10 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
     E* vecs = Ms + (1 + nw) * NP * S;           // XD0[NP], D[NP] (loop)
     E* XD0 = vecs;
     E* Dv = vecs + NP;
-    cplx* small = reinterpret_cast<cplx*>(vecs + 2 * NP) + wave * (4 * (WH_EMAX + 2));
-    cplx* pt = small;                            // p[0..h]
-    cplx* lv = pt + (WH_EMAX + 2);               // loop scalars v[1..h]
-    E* comb0 = reinterpret_cast<E*>(lv + (WH_EMAX + 2));
-    E* comb1 = comb0 + (WH_EMAX + 2);
-    double4* wres = reinterpret_cast<double4*>(reinterpret_cast<cplx*>(vecs + 2 * NP) +
-                                               nw * (4 * (WH_EMAX + 2)));
+    constexpr int PTN = pm1_ptn(T);
+    constexpr int NB = pm1_nb(T, REAL);
+    cplx* slots = reinterpret_cast<cplx*>(vecs + 2 * NP) + wave * (NB * 2 * PTN);
+    double4* wres = reinterpret_cast<double4*>(reinterpret_cast<cplx*>(vecs + 2 * NP) + nw * (NB * 2 * PTN));
 
     // ---- stage M = Ax X (column c of M is column (c + h) mod n of Ax), zero padded
     const cplx* A = pool + J.a_off;
@@ -317,8 +320,12 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
     csum acc_sum;
     acc_sum.zero();
 
+    int slot = 0;
+    int64_t t_batch = t0 + wave;  // the batch's first term
     for (int64_t t = t0 + wave; t < t1; t += nw) {
         const uint64_t tu = (uint64_t)t;
+        cplx* pt = slots + slot * 2 * PTN;  // p[0..h]
+        cplx* lv = pt + PTN;                // loop scalars v[1..h]
         double sgnJ[T];
 #pragma unroll
         for (int c = 0; c < T; ++c) sgnJ[c] = col_sign(tu, J0 + c, h, unset);
@@ -394,7 +401,7 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
                 const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
                 if (lane == 0) {
                     pt[ko] = qoc;
-                    pt[ke] = qec;  // ke <= h + 1 < WH_EMAX + 2
+                    pt[ke] = qec;  // ke <= h + 1 < PTN
                 }
                 if (LOOP) {
                     E y[T], z[T], xdI[T], dJ[T];
@@ -443,14 +450,43 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
             continue;
         }
         // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2): the
-        // reference builds it factor by factor (_hafnian.py:200-208, :232-241); every lane runs
-        // the power-series exponential e_k = (1/k) sum_(i<=k) (i a_i) e_(k-i) in registers
-        const E top = coeff_top_e<E, 4 * T, LOOP>(pt, lv, h);
-        if (lane == h) acc_sum.add(cmulr(etoc(top), prefac));
+        // reference builds it factor by factor (_hafnian.py:200-208, :232-241); here lane j runs
+        // the power-series exponential e_k = (1/k) sum_(i<=k) (i a_i) e_(k-i) in registers for
+        // the j-th term of a batch of up to NB terms
+        ++slot;
+        if (slot == NB || t + nw >= t1) {
+            wave_lds_sync();  // pt and lv of every slot as lane 0 stored them
+            if (lane < slot) {
+                const cplx* ptj = slots + lane * 2 * PTN;
+                const uint64_t tj = (uint64_t)(t_batch + (int64_t)lane * nw);
+                const double pj = ((h - __popcll(tj)) & 1) ? -1.0 : 1.0;
+                const E top = coeff_top_e<E, 4 * T, LOOP>(ptj, ptj + PTN, h);
+                acc_sum.add(cmulr(etoc(top), pj));
+            }
+            wave_lds_sync();  // the slots are free again
+            slot = 0;
+            t_batch = t + nw;
+        }
     }
 
-    // fold the waves of this workgroup in wave order
-    if (lane == h) wres[wave] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    // fold the lanes' partial sums in lane order (montrealer: lane h holds them all), then the
+    // waves of this workgroup in wave order
+    if (mtl) {
+        if (lane == h) wres[wave] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    } else {
+        double4* lsum = reinterpret_cast<double4*>(slots);
+        if (lane < NB) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+        wave_lds_sync();
+        if (lane == 0) {
+            csum s;
+            s.zero();
+            for (int j = 0; j < NB; ++j) {
+                double4 pq = lsum[j];
+                s.add_pair(pq.x, pq.y, pq.z, pq.w);
+            }
+            wres[wave] = make_double4(s.sr, s.cr, s.si, s.ci);
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         csum s;
@@ -483,7 +519,8 @@ size_t wh_pm1_lds_bytes(int T, bool real, int nw) {
     size_t esz = real ? sizeof(double) : sizeof(cplx);
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1_PAD;
     size_t mat = (NP * S * esz + 31) / 32 * 32;  // keeps the later regions 32-byte aligned
-    return (1 + nw) * mat + 2 * NP * esz + nw * 4 * (WH_EMAX + 2) * sizeof(cplx) + nw * sizeof(double4) + 32;
+    return (1 + nw) * mat + 2 * NP * esz + nw * (size_t)(pm1_nb(T, real) * 2 * pm1_ptn(T)) * sizeof(cplx) +
+           nw * sizeof(double4) + 32;
 }
 
 template <int T, bool REAL, bool LOOP, bool HALF>
