@@ -209,7 +209,7 @@ def next_rows_run(with_cpu):
 
 # HBM bytes per launch of the dominant kernel on the n in [34, 40] loop sub-batch of the
 # default stream: FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3, scaled by the copy calibration
-TRAFFIC_PM1S_T5_LOOP = 4.40e6
+TRAFFIC_PM1S_T5_LOOP = 4.41e6
 
 
 def mfma_flops_executed(n):
