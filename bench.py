@@ -105,8 +105,9 @@ def cpu_baseline(reqs, budget_s=20.0):
     }
 
 
-def configs_run(torch, tw, B, stream):
-    """Configurations 1-4, one measurement each (inputs resident for the batched ones)."""
+def configs_run(torch, tw, B, stream, with_cpu=True):
+    """Configurations 1-4, one measurement each (inputs resident for the batched ones); with
+    ``with_cpu`` the n = 24 batch is also timed on the CPU oracle over a bounded sample."""
     out = {}
     A1 = W.config1()
     tw.hafnian(A1)
@@ -126,6 +127,20 @@ def configs_run(torch, tw, B, stream):
         "all_ones_exact": bool(all(round(v.real) == 316234143225 for v in r2[-16:])),
     }
     b2.close()
+    if with_cpu:
+        # the same real n = 24 matrices on the CPU oracle (all host threads), 32 of them
+        from oracle import oracle as O
+
+        cores = len(os.sched_getaffinity(0))
+        O.hafnian_sieve(mats[0], nthreads=cores)
+        t0 = time.perf_counter()
+        for M in mats[:32]:
+            O.hafnian_sieve(M, nthreads=cores)
+        cpu_rate = 32 / (time.perf_counter() - t0)
+        row = out["config2_real24_batch"]
+        row["cpu_oracle_hafnians_per_s"] = cpu_rate
+        row["cpu_cores"] = cores
+        row["gpu_over_cpu"] = row["hafnians_per_s"] / cpu_rate
 
     A3 = W.config3()
     tw.hafnian(A3, loop=True)
@@ -342,7 +357,7 @@ def main():
                  "graph_equals_eager": bool(np.array_equal(sb.results(), r_eager))}
         sb.close()
 
-    configs = None if args.no_configs else configs_run(torch, tw, B, stream)
+    configs = None if args.no_configs else configs_run(torch, tw, B, stream, with_cpu=not args.no_cpu)
     next_rows = None if args.no_next else next_rows_run(not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(reqs)
     head = serving["load_0.9"] if serving else {}
