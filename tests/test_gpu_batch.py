@@ -241,3 +241,25 @@ def test_tuned_chunking_changes_chunks_not_results():
     assert cb[0] < ca[0] and cb[1] == ca[1]
     assert relerr(ra[0], rb[0]) < 1e-12
     assert ra[1] == rb[1]
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_odd_chunk_sizes_agree(loop):
+    """Chunks of 1, 3, 5 and 13 terms (waves of a workgroup with unequal term counts, partial
+    coefficient batches) give the default chunking's results to rounding, on the symmetric
+    matrix-core kernel and on the vector kernel."""
+    rng = np.random.default_rng(81 + loop)
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in (10, 16, 22, 26)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    for kw in ({}, {"valu_products": True}):
+        ref = B.DeviceBatch(preps, **kw)
+        ref.run()
+        r0 = ref.results()
+        ref.close()
+        for ct in (1, 3, 5, 13):
+            db = B.DeviceBatch(preps, chunk_terms=ct, **kw)
+            db.run()
+            r = db.results()
+            db.close()
+            for k in range(len(reqs)):
+                assert relerr(r[k], r0[k]) < 1e-12, (kw, ct, reqs[k][0].shape[0], relerr(r[k], r0[k]))
