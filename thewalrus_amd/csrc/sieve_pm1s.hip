@@ -15,7 +15,9 @@
 // a summand symmetric in (i, k): the upper groups off the diagonal count twice and the row sign
 // d_i scales each row group's partial sum once.  The loop vectors z_a = B^a D follow the chain
 // z_a = A (X D_d z_(a-1)), and the loop dots use y_a = X D_d z_a as in sieve_pm1m.hip.  The
-// montrealer (0/1 factors: D_d^2 is not I) stays on sieve_pm1m.hip.
+// montrealer (0/1 factors: D_d^2 is not I) stays on sieve_pm1m.hip.  Each wave keeps the power
+// traces and loop scalars of its last few terms in LDS and forms their coefficients together,
+// one term per lane.
 //
 // Lane map and MFMA layout as sieve_pm1m.hip: x = lane & 3, br = bit 2, bc = bit 3,
 // y = lane >> 4; group (p, q) holds R[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
