@@ -166,8 +166,10 @@ int describe(const wh_request& r, wh_job_dev& j) {
     return WH_OK;
 }
 
-// Kernel families.  PM1: Glynn, even N, every multiplicity 1 (m = n for every term); the
-// register-tiled kernel with T x T tiles per lane, n_pad = 8T.  GENERIC: everything else.
+// Kernel families.  The all-ones kernels take Glynn terms with even N and every multiplicity
+// 1 (m = n for every term), n_pad = 8T: PM1 the register-tiled vector kernel (T x T tiles per
+// lane), PM1S the symmetric-power matrix-core kernel, PM1M the full-power matrix-core kernel.
+// GENERIC: everything else.
 enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2, KIND_PM1S = 3 };
 static bool all_ones_kind(int k) { return k == KIND_PM1 || k == KIND_PM1M || k == KIND_PM1S; }
 constexpr int kPm1MaxT = 6;
