@@ -82,3 +82,29 @@ def test_oversized_request_rejected():
     with Server(slots=1, slot_pool_mb=0.01) as srv:
         with pytest.raises(ValueError):
             srv.submit(np.ones((30, 30)))
+
+
+def test_queue_deadline_and_cancel():
+    """With the only slot busy on an n = 40 request, a queued request past its deadline fails
+    with TimeoutError, a cancelled one is dropped, and the others complete with their values
+    (deadlines are host-side: nothing is interrupted on the device)."""
+    rng = np.random.default_rng(91)
+    big = random_symmetric(rng, 40, scale=1 / np.sqrt(40))
+    small = [random_symmetric(rng, 10, scale=1 / np.sqrt(10)) for _ in range(3)]
+    with Server(slots=1, slot_pool_mb=2) as srv:
+        f_big = srv.submit(big, loop=True)
+        t0 = time.perf_counter()
+        while srv.stats.batches < 1 and time.perf_counter() - t0 < 30:
+            time.sleep(0.0002)
+        assert srv.stats.batches == 1  # the big request holds the only slot
+        f_late = srv.submit(small[0], deadline_s=0.0)
+        f_cancel = srv.submit(small[1])
+        assert f_cancel.cancel()
+        f_ok = srv.submit(small[2], deadline_s=120.0)
+        assert srv.drain(timeout=120)
+        with pytest.raises(TimeoutError):
+            f_late.result()
+        assert f_cancel.cancelled()
+        assert f_ok.result() == tw.hafnian(small[2])  # same request, same kernel and chunking
+        assert f_big.result() == tw.hafnian(big, loop=True)
+        assert srv.stats.expired == 1 and srv.stats.cancelled == 1

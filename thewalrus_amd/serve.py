@@ -40,14 +40,15 @@ def _partial_slots(lib, job, chunk_terms):
 
 
 class _Item:
-    __slots__ = ("job", "future", "t_submit", "t_launch", "t_done", "elems", "slots", "value")
+    __slots__ = ("job", "future", "t_submit", "t_launch", "t_done", "elems", "slots", "value", "t_deadline")
 
-    def __init__(self, job, future, elems, slots, t_submit):
+    def __init__(self, job, future, elems, slots, t_submit, t_deadline=None):
         self.job, self.future = job, future
         self.elems, self.slots = elems, slots
         self.t_submit = t_submit
         self.t_launch = self.t_done = None
         self.value = None
+        self.t_deadline = t_deadline
 
 
 class ServerStats:
@@ -57,6 +58,8 @@ class ServerStats:
         self.submit, self.done, self.launch = [], [], []
         self.batches = 0
         self.batch_sizes = []
+        self.cancelled = 0  # dropped from the queue after Future.cancel()
+        self.expired = 0    # failed with TimeoutError: not launched within their deadline
 
     def latencies_ms(self):
         return (np.asarray(self.done) - np.asarray(self.submit)) * 1e3
@@ -135,10 +138,15 @@ class Server:
         self._thread.start()
 
     # ---------------------------------------------------------------- submission
-    def submit(self, A, loop=False, rpt=None, mu=None, glynn=True, method="glynn", rtol=1e-05, atol=1e-08):
+    def submit(self, A, loop=False, rpt=None, mu=None, glynn=True, method="glynn", rtol=1e-05, atol=1e-08,
+               deadline_s=None):
         """Queue one request; returns a Future with the value (numpy complex128, or the
         frontend's exact value for shortcut cases).  Validation errors raise immediately.
-        Latency is measured from entry to this call (host preparation included)."""
+        Latency is measured from entry to this call (host preparation included).
+
+        ``deadline_s``: a request still queued (not launched) that long after submission fails
+        with ``TimeoutError`` instead; a launched request always completes.  A request whose
+        future is cancelled (``Future.cancel()``) before launch is dropped from the queue."""
         t_submit = time.perf_counter()
         if rpt is None:
             r = H.prepare_hafnian(A, loop=loop, rtol=rtol, atol=atol, method=method)
@@ -148,7 +156,8 @@ class Server:
         if not isinstance(r, H.SieveJob):
             fut.set_result(r)
             return fut
-        item = _Item(r, fut, job_elements(r), _partial_slots(self._lib, r, self.chunk_terms), t_submit)
+        item = _Item(r, fut, job_elements(r), _partial_slots(self._lib, r, self.chunk_terms), t_submit,
+                     None if deadline_s is None else t_submit + float(deadline_s))
         slot0 = self.slots[0]
         if item.elems > slot0.pool_elems or item.slots > slot0.partial_slots:
             raise ValueError("request exceeds the per-slot arena (%d elements, %d slots)" % (item.elems, item.slots))
@@ -211,6 +220,23 @@ class Server:
         slot.items = slot.batch = None
         slot.footprint = 0
 
+    def _expire(self):
+        """Drops queued requests that were cancelled or whose queue deadline has passed (the
+        caller holds the lock)."""
+        now = time.perf_counter()
+        keep = collections.deque()
+        for it in self.queue:
+            if it.future.cancelled():
+                self.pending -= 1
+                self.stats.cancelled += 1
+            elif it.t_deadline is not None and now > it.t_deadline:
+                self.pending -= 1
+                self.stats.expired += 1
+                it.future.set_exception(TimeoutError("request not launched within its deadline"))
+            else:
+                keep.append(it)
+        self.queue = keep
+
     def _take(self, slot):
         """Requests for one batch: arrival order, within the slot arena and the HBM budget."""
         items, elems, parts, fp = [], 0, 0, 0
@@ -221,6 +247,11 @@ class Server:
                     or self.admitted_bytes + fp + f > self.hbm_budget):
                 break
             self.queue.popleft()
+            # RUNNING from here on (no longer cancellable); False: cancelled while queued
+            if not it.future.set_running_or_notify_cancel():
+                self.pending -= 1
+                self.stats.cancelled += 1
+                continue
             items.append(it)
             elems += it.elems
             parts += it.slots
@@ -242,6 +273,8 @@ class Server:
                 if self._stop and self.pending == 0:
                     return
                 launch = []
+                if self.queue:
+                    self._expire()
                 while self.free and self.queue:
                     slot = self.free[0]
                     items, fp = self._take(slot)
@@ -252,7 +285,7 @@ class Server:
                     self.admitted_bytes += fp
                     launch.append((slot, items))
                 if not launch and not progressed:
-                    self._wake.wait(timeout=0.0002 if self.busy else 0.05)
+                    self._wake.wait(timeout=0.0002 if (self.busy or self.queue) else 0.05)
             for slot, items in launch:
                 try:
                     self._launch(slot, items)
