@@ -151,6 +151,22 @@ int wh_batch_write_jobs(const wh_batch* batch, void* host_dst);
 int wh_batch_launch(const wh_batch* batch, const void* h_jobs, void* d_jobs, const double* dpool,
                     double* d_partials, double* dout, void* stream);
 
+/* The serving loop's form of write_jobs + launch, one call per batch (no counterpart in the
+ * reference: its requests run one numba call each).  Checks the arenas (job table bytes in
+ * h_jobs / d_jobs, pool_elems complex128 in d_pool, partial slots, n results), then writes the
+ * job table to the pinned h_jobs and enqueues on `stream`: pool h_pool -> d_pool, the job
+ * table, the sieve kernels and the fold, and the n results d_out -> h_out (pinned; complex128,
+ * request order).  Nothing is enqueued when a check fails.  Pinned buffers stay unchanged (and
+ * h_out unread) until the stream passes this point; buffers as in wh_batch_launch. */
+int wh_batch_submit(const wh_batch* batch, void* h_jobs, void* d_jobs, size_t jobs_capacity_bytes,
+                    const double* h_pool, double* d_pool, int64_t pool_elems, double* d_partials,
+                    int64_t partial_capacity, double* d_out, double* h_out, int64_t out_capacity,
+                    void* stream);
+
+/* Partial-sum slots one request takes (its share of any batch: the layout of a request
+ * depends on the request alone).  Host only. */
+int wh_request_partials(const wh_request* req, int64_t chunk_terms, int64_t* partial_slots);
+
 /* Per request (request order): first partial slot, number of slots, terms per slot. */
 int wh_batch_chunks(const wh_batch* batch, int64_t* first_chunk, int64_t* nchunks,
                     int64_t* chunk_terms);

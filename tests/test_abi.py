@@ -50,3 +50,57 @@ def test_error_string_without_device():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError):
         _native.load(str(tmp_path / "nope.so"))
+
+
+def test_request_partials_match_single_batch_layout():
+    """wh_request_partials (one call per served request) equals the partial slots of a batch of
+    that request alone; both are host-only layout calls."""
+    import numpy as np
+
+    from thewalrus_amd import _hafnian as H
+    from thewalrus_amd import batch as B
+
+    L = _native.load()
+    rng = np.random.default_rng(7)
+    jobs = []
+    for n, loop in ((6, False), (9, True), (20, True), (24, False), (40, True)):
+        X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        jobs.append(H.prepare_hafnian((X + X.T) / 2, loop=loop))
+    X = rng.normal(size=(6, 6))
+    jobs.append(H.prepare_hafnian_repeated((X + X.T) / 2, [3, 1, 2, 0, 1, 2], mu=np.arange(6) + 0.5, loop=True))
+    for job in jobs:
+        pk = B.pack_one(job)
+        for chunk in (0, 7):
+            got = ctypes.c_int64()
+            assert L.wh_request_partials(ctypes.byref(pk.req), chunk, ctypes.byref(got)) == 0
+            b = ctypes.c_void_p()
+            assert L.wh_batch_create(ctypes.byref(pk.req), 1, chunk, ctypes.byref(b)) == 0
+            want = ctypes.c_int64()
+            assert L.wh_batch_info(b, None, ctypes.byref(want), None, None) == 0
+            L.wh_batch_destroy(b)
+            assert got.value == want.value > 0
+
+
+def test_batch_submit_checks_arenas_before_any_device_call():
+    """A batch larger than the caller's arenas is refused with a message, before any copy or
+    launch (so this runs without a device)."""
+    import numpy as np
+
+    from thewalrus_amd import _hafnian as H
+    from thewalrus_amd import batch as B
+
+    L = _native.load()
+    X = np.random.default_rng(8).normal(size=(12, 12))
+    pk = B.pack_one(H.prepare_hafnian((X + X.T) / 2))
+    b = ctypes.c_void_p()
+    assert L.wh_batch_create(ctypes.byref(pk.req), 1, 0, ctypes.byref(b)) == 0
+    try:
+        fake = 1 << 12  # aligned, never dereferenced: the capacity checks fail first
+        rc = L.wh_batch_submit(b, fake, fake, 0, fake, fake, pk.elems, fake, 1 << 20, fake, fake, 1, None)
+        assert rc != 0 and b"arenas" in L.wh_last_error()
+        rc = L.wh_batch_submit(b, fake, fake, 1 << 20, fake, fake, pk.elems - 1, fake, 1 << 20, fake, fake, 1, None)
+        assert rc != 0 and b"pool" in L.wh_last_error()
+        rc = L.wh_batch_submit(b, None, fake, 1 << 20, fake, fake, pk.elems, fake, 1 << 20, fake, fake, 1, None)
+        assert rc != 0
+    finally:
+        L.wh_batch_destroy(b)

@@ -167,3 +167,66 @@ class TestPreparedJobs:
         assert not p.on_device and p.value == 3.0
         p = B.prepare(np.ones((6, 6)))
         assert p.on_device
+
+
+class TestHostFastPaths:
+    """The shortcuts of the host checks decide exactly as numpy.allclose does."""
+
+    @staticmethod
+    def _cases(rng):
+        out = []
+        for n in (1, 2, 3, 6, 9):
+            S = random_symmetric(rng, n)
+            out.append(S)
+            for scale in (1e-12, 1e-9, 1e-8, 1e-7, 1e-5, 1e-3):  # around atol and rtol
+                P = S.copy()
+                if n > 1:
+                    P[0, 1] += scale
+                out.append(P)
+            D = np.diag(rng.normal(size=n)).astype(np.complex128)
+            for off in (0.0, 5e-9, 1e-8, 1.0000001e-8, 2e-8, 1e-3):
+                Q = D.copy()
+                if n > 1:
+                    Q[0, 1] = Q[1, 0] = off
+                out.append(Q)
+                out.append(Q.real.copy())
+            out.append(np.round(S.real * 4).astype(np.int64))
+            out.append(S.astype(np.complex64))
+            out.append(S.real.astype(np.float32))
+            out.append(np.eye(n, dtype=bool))
+        I = np.diag([1.0, 2.0, 3.0])
+        I[0, 1] = np.inf
+        out.append(I)
+        I2 = I.copy()
+        I2[1, 0] = np.inf
+        out.append(I2)
+        return out
+
+    @pytest.mark.parametrize("tol", [(1e-05, 1e-08), (0.0, 0.0), (0.5, 1e-3), (2.0, 0.0)])
+    def test_symmetry_and_diagonal_decisions(self, rng, tol):
+        rtol, atol = tol
+        for A in self._cases(rng):
+            want = np.allclose(A, A.T, rtol=rtol, atol=atol)
+            try:
+                H.input_validation(A, rtol=rtol, atol=atol)
+                got = True
+            except ValueError:
+                got = False
+            assert got == want, (A, tol)
+            assert H._is_diagonal(A, rtol, atol) == np.allclose(np.diag(np.diag(A)), A, rtol=rtol, atol=atol)
+
+    def test_all_ones_pairing_cache(self):
+        for n in range(0, 70):
+            a, b = tw.matched_reps([1] * n), H._pairing([1] * n, True)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+        x, e, _ = H._pairing([1] * 8, True)
+        e[0] = 7  # a caller's copy: the cache is unchanged
+        assert list(H._pairing([1] * 8, True)[1]) == [1] * 4
+
+    def test_reorder_equals_fancy_index(self, rng):
+        for n in (2, 5, 8):
+            x = rng.permutation(n)
+            for A in (random_symmetric(rng, n), rng.normal(size=(n, n)), rng.integers(-5, 5, size=(n, n))):
+                R = H._reorder(A, x)
+                assert R.dtype == np.complex128 and R.flags.c_contiguous
+                assert np.array_equal(R, A[np.ix_(x, x)].astype(np.complex128))

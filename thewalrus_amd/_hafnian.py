@@ -6,6 +6,7 @@ Same names, signatures, defaults, exceptions and return types as ``thewalrus._ha
 closed-form shortcuts for n <= 4 and the mode pairing run on the host exactly as in the
 reference; every sieve evaluation runs on the GPU through ``libwalrus_hip.so``.
 """
+import functools
 import warnings
 
 import numpy as np
@@ -27,9 +28,32 @@ def input_validation(A, rtol=1e-05, atol=1e-08):
         raise ValueError("Input matrix must be square.")
     if np.isnan(A).any():
         raise ValueError("Input matrix must not contain NaNs.")
-    if not np.allclose(A, A.T, rtol=rtol, atol=atol):
+    # exactly equal entries are close under any tolerance (numpy.isclose: ... or x == y), so
+    # the tolerance test only runs for a matrix that is not exactly symmetric
+    exact = A.dtype.kind in "biufc" and bool((A == A.T).all())
+    if not exact and not np.allclose(A, A.T, rtol=rtol, atol=atol):
         raise ValueError("Input matrix must be symmetric.")
     return True
+
+
+def _surely_not_diagonal(A, rtol, atol):
+    """True when the off-diagonal entry A[0, 1] is far from close to 0 under numpy.isclose's rule
+    (|0 - y| <= atol + rtol |y|, y finite): then ``np.allclose(diag(diag(A)), A)`` is False.  The
+    margin covers the last-bit differences between numpy's array and scalar complex moduli; a
+    False here decides nothing (the full test runs)."""
+    if A.shape[0] < 2 or A.dtype.kind not in "fc" or np.ndim(rtol) or np.ndim(atol):
+        return False
+    ay = abs(complex(A[0, 1]))
+    if not ay < float("inf"):
+        return False
+    return ay * (1.0 - 1e-9) > float(atol) + float(rtol) * ay * (1.0 + 1e-9)
+
+
+def _is_diagonal(A, rtol, atol):
+    """``np.allclose(np.diag(np.diag(A)), A)`` (``_hafnian.py:766``)."""
+    if _surely_not_diagonal(A, rtol, atol):
+        return False
+    return np.allclose(np.diag(np.diag(A)), A, rtol=rtol, atol=atol)
 
 
 def reduction(A, rpt):
@@ -105,10 +129,34 @@ class SieveJob:
                                          self.glynn)
 
 
+@functools.lru_cache(maxsize=None)
+def _matched_ones(n):
+    """``matched_reps([1] * n)``, computed once per n."""
+    x, e, odd = matched_reps([1] * n)
+    x.flags.writeable = False
+    e.flags.writeable = False
+    return x, e, odd
+
+
+def _pairing(reps, ones):
+    """``matched_reps(reps)``; every multiplicity 1 (``ones``) comes from the per-n cache."""
+    if not ones:
+        return matched_reps(reps)
+    x, e, odd = _matched_ones(len(reps))
+    return x, e.copy(), odd
+
+
+def _reorder(A, x):
+    """``A[np.ix_(x, x)].astype(np.complex128)`` (``_hafnian.py:505, :627``), as two gathers: a new
+    C-contiguous complex128 array."""
+    return A.take(x, axis=0).take(x, axis=1).astype(np.complex128, copy=False)
+
+
 def _prepare_haf(A, reps=None, glynn=True):
     """Host part of ``_haf`` (``_hafnian.py:486-505``): a value, or the sieve job."""
     n = A.shape[0]
-    if reps is None:
+    ones = reps is None
+    if ones:
         reps = [1] * n
     N = sum(reps)
     if N == 0:
@@ -116,15 +164,15 @@ def _prepare_haf(A, reps=None, glynn=True):
     if N % 2 == 1:
         return 0.0
     assert n == len(reps)
-    x, edge_reps, _ = matched_reps(reps)
-    Ax = A[np.ix_(x, x)].astype(np.complex128)
-    return SieveJob(Ax, edge_reps, glynn=glynn)
+    x, edge_reps, _ = _pairing(reps, ones)
+    return SieveJob(_reorder(A, x), edge_reps, glynn=glynn)
 
 
 def _prepare_loop(A, D=None, reps=None, glynn=True):
     """Host part of ``loop_hafnian`` (``_hafnian.py:598-628``): a value, or the sieve job."""
     n = A.shape[0]
-    if reps is None:
+    ones = reps is None
+    if ones:
         reps = [1] * n
     if D is None:
         D = A.diagonal()
@@ -135,16 +183,15 @@ def _prepare_loop(A, D=None, reps=None, glynn=True):
         return D[np.where(np.array(reps) == 1)[0][0]]
     assert n == len(reps)
     assert D.shape[0] == n
-    x, edge_reps, oddmode = matched_reps(reps)
+    x, edge_reps, oddmode = _pairing(reps, ones)
     if oddmode is not None:
         oddloop = D[oddmode].astype(np.complex128)
         oddV = A[oddmode, x].astype(np.complex128)
     else:
         oddloop = None
         oddV = None
-    Ax = A[np.ix_(x, x)].astype(np.complex128)
     Dx = D[x].astype(np.complex128)
-    return SieveJob(Ax, edge_reps, Dx, oddloop, oddV, glynn)
+    return SieveJob(_reorder(A, x), edge_reps, Dx, oddloop, oddV, glynn)
 
 
 def _finish(r):
@@ -212,7 +259,7 @@ def prepare_hafnian(A, loop=False, rtol=1e-05, atol=1e-08, approx=False, num_sam
         return 1
     if shape[0] % 2 != 0 and not loop:
         return 0.0
-    if np.allclose(np.diag(np.diag(A)), A, rtol=rtol, atol=atol):
+    if _is_diagonal(A, rtol, atol):
         if loop:
             return np.prod(np.diag(A))
         return 0

@@ -46,6 +46,8 @@ EXPORTS = [
     "wh_batch_info",
     "wh_batch_write_jobs",
     "wh_batch_launch",
+    "wh_batch_submit",
+    "wh_request_partials",
     "wh_batch_chunks",
     "wh_batch_destroy",
     "wh_plan_create",
@@ -119,6 +121,11 @@ def load(path=None):
         L.wh_batch_write_jobs.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.wh_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_batch_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                      ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p]
+        L.wh_request_partials.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, _ip]
         L.wh_batch_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         L.wh_batch_destroy.argtypes = [ctypes.c_void_p]
         L.wh_plan_partials.argtypes = [ctypes.c_void_p, _dp]

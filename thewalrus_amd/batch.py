@@ -45,13 +45,85 @@ def prepare_repeated(A, rpt, mu=None, loop=False, rtol=1e-05, atol=1e-08, glynn=
 
 
 def _is_real(*arrays):
-    return all(a is None or not np.iscomplexobj(a) or not np.any(np.imag(a)) for a in arrays)
+    """No array has a nonzero imaginary part (None entries are skipped)."""
+    for a in arrays:
+        if a is None or not np.iscomplexobj(a) or a.size == 0:
+            continue
+        if a.flat[a.size // 2].imag != 0 or a.imag.any():  # one entry first: most inputs stop there
+            return False
+    return True
 
 
 def job_elements(job):
     """complex128 elements a sieve job occupies in a device pool."""
     twoE = job.Ax.shape[0]
     return twoE * twoE + (twoE if job.loop else 0) + (twoE if job.oddloop is not None else 0)
+
+
+class PackedJob:
+    """A sieve job in its device form: one complex128 ``payload`` ([Ax | Dx | oddV], the parts
+    the job has) and its ``wh_request`` with offsets relative to the payload's start."""
+
+    __slots__ = ("payload", "req")
+
+    def __init__(self, payload, req):
+        self.payload, self.req = payload, req
+
+    @property
+    def elems(self):
+        return self.payload.shape[0]
+
+
+def pack_one(job, options=0):
+    """The device form of one sieve job (see :class:`PackedJob`)."""
+    twoE = job.Ax.shape[0]
+    E = twoE // 2
+    if E > _native.MAX_EDGES:
+        raise ValueError("%d edges exceed the device limit %d" % (E, _native.MAX_EDGES))
+    r = _native.WhRequest()
+    r.E = E
+    r.glynn = int(bool(job.glynn))
+    r.options = options
+    r.edge_reps[:E] = np.asarray(job.edge_reps, dtype=np.int64).tolist()
+    r.a_off = 0
+    r.d_off = r.v_off = -1
+    parts = [job.Ax.ravel()]
+    if job.loop:
+        r.d_off = twoE * twoE
+        parts.append(job.Dx)
+        if job.oddloop is not None:
+            r.v_off = twoE * twoE + twoE
+            parts.append(job.oddV)
+            r.oddloop[0] = complex(job.oddloop).real
+            r.oddloop[1] = complex(job.oddloop).imag
+    r.real = int(job.oddloop is None and _is_real(job.Ax, job.Dx))
+    payload = np.concatenate(parts).astype(np.complex128, copy=False) if len(parts) > 1 else \
+        np.ascontiguousarray(parts[0], dtype=np.complex128)
+    return PackedJob(payload, r)
+
+
+def layout_packed(packed, pool):
+    """Lays packed jobs out one after another in ``pool`` (a writable complex128 array).
+
+    Returns (requests, used): a ctypes ``wh_request`` array with offsets into ``pool`` and the
+    used part of the pool."""
+    total = sum(p.elems for p in packed)
+    if pool.shape[0] < total:
+        raise ValueError("pool holds %d elements, %d needed" % (pool.shape[0], total))
+    reqs = (_native.WhRequest * max(len(packed), 1))()
+    off = 0
+    for k, p in enumerate(packed):
+        reqs[k] = p.req
+        r = reqs[k]
+        r.a_off = off
+        if r.d_off >= 0:
+            r.d_off += off
+        if r.v_off >= 0:
+            r.v_off += off
+        n = p.elems
+        pool[off:off + n] = p.payload
+        off += n
+    return reqs, pool[:max(total, 1)]
 
 
 def pack_jobs(jobs, pool=None, options=0):
@@ -61,40 +133,10 @@ def pack_jobs(jobs, pool=None, options=0):
     ``pool`` (a writable complex128 array, e.g. a view of pinned memory) is given, the data
     is written into it; otherwise a new array is allocated.
     """
-    total = sum(job_elements(j) for j in jobs)
+    packed = [pack_one(j, options) for j in jobs]
     if pool is None:
-        pool = np.empty(max(total, 1), dtype=np.complex128)
-    elif pool.shape[0] < total:
-        raise ValueError("pool holds %d elements, %d needed" % (pool.shape[0], total))
-    reqs = (_native.WhRequest * max(len(jobs), 1))()
-    off = 0
-    for k, p in enumerate(jobs):
-        r = reqs[k]
-        twoE = p.Ax.shape[0]
-        E = twoE // 2
-        if E > _native.MAX_EDGES:
-            raise ValueError("%d edges exceed the device limit %d" % (E, _native.MAX_EDGES))
-        r.E = E
-        r.glynn = int(bool(p.glynn))
-        r.options = options
-        for i in range(E):
-            r.edge_reps[i] = int(p.edge_reps[i])
-        r.a_off = off
-        pool[off:off + twoE * twoE] = np.asarray(p.Ax, dtype=np.complex128).ravel()
-        off += twoE * twoE
-        r.d_off = r.v_off = -1
-        if p.loop:
-            r.d_off = off
-            pool[off:off + twoE] = p.Dx
-            off += twoE
-            if p.oddloop is not None:
-                r.v_off = off
-                pool[off:off + twoE] = p.oddV
-                off += twoE
-                r.oddloop[0] = complex(p.oddloop).real
-                r.oddloop[1] = complex(p.oddloop).imag
-        r.real = int(p.oddloop is None and _is_real(p.Ax, p.Dx))
-    return reqs, pool[:max(total, 1)]
+        pool = np.empty(max(sum(p.elems for p in packed), 1), dtype=np.complex128)
+    return layout_packed(packed, pool)
 
 
 class DeviceBatch:

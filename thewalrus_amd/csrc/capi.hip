@@ -300,6 +300,15 @@ struct Group {
     int MM = 0;  // generic: LDS matrix side
 };
 
+// One past the last pool element a job reads: the 2E x 2E matrix and the 2E-vectors D and V.
+int64_t job_pool_end(const wh_job_dev& j) {
+    const int64_t side = 2 * (int64_t)j.E;
+    int64_t end = j.a_off + side * side;
+    if (j.d_off >= 0) end = std::max(end, j.d_off + side);
+    if (j.v_off >= 0) end = std::max(end, j.v_off + side);
+    return end;
+}
+
 struct Layout {
     std::vector<wh_job_dev> jobs;  // sorted by group
     std::vector<Group> groups;
@@ -451,6 +460,44 @@ int wh_batch_launch(const wh_batch* b, const void* h_jobs, void* d_jobs, const d
         HIP_TRY(hipMemcpyAsync(d_jobs, h_jobs, b->layout.jobs.size() * sizeof(wh_job_dev),
                                hipMemcpyHostToDevice, s));
     HIP_TRY(run_layout(b->layout, (const wh_job_dev*)d_jobs, dpool, (double4*)d_partials, dout, s));
+    return WH_OK;
+}
+
+int wh_batch_submit(const wh_batch* b, void* h_jobs, void* d_jobs, size_t jobs_capacity_bytes,
+                    const double* h_pool, double* d_pool, int64_t pool_elems, double* d_partials,
+                    int64_t partial_capacity, double* d_out, double* h_out, int64_t out_capacity,
+                    void* stream) {
+    if (!b || !h_jobs || !d_jobs || !h_pool || !d_pool || !d_partials || !d_out || !h_out)
+        return fail(WH_EINVAL, "null batch or buffer");
+    if (((uintptr_t)d_jobs | (uintptr_t)d_partials) & 31)
+        return fail(WH_EINVAL, "job table and partials must be 32-byte aligned");
+    const Layout& L = b->layout;
+    const size_t jb = L.jobs.size() * sizeof(wh_job_dev);
+    const int64_t nout = (int64_t)L.jobs.size();
+    int64_t need_pool = 0;
+    for (const wh_job_dev& j : L.jobs) need_pool = std::max(need_pool, job_pool_end(j));
+    if (jb > jobs_capacity_bytes || L.blocks > partial_capacity || nout > out_capacity || need_pool > pool_elems)
+        return fail(WH_EINVAL,
+                    "batch exceeds the arenas (job table %zu of %zu B, %lld of %lld partial slots, %lld of %lld "
+                    "results, pool %lld of %lld elements)",
+                    jb, jobs_capacity_bytes, (long long)L.blocks, (long long)partial_capacity, (long long)nout,
+                    (long long)out_capacity, (long long)need_pool, (long long)pool_elems);
+    hipStream_t s = (hipStream_t)stream;
+    memcpy(h_jobs, L.jobs.data(), jb);
+    if (pool_elems > 0)
+        HIP_TRY(hipMemcpyAsync(d_pool, h_pool, (size_t)pool_elems * 16, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, s));
+    HIP_TRY(run_layout(L, (const wh_job_dev*)d_jobs, d_pool, (double4*)d_partials, d_out, s));
+    HIP_TRY(hipMemcpyAsync(h_out, d_out, (size_t)nout * 16, hipMemcpyDeviceToHost, s));
+    return WH_OK;
+}
+
+int wh_request_partials(const wh_request* req, int64_t chunk_terms, int64_t* partial_slots) {
+    if (!req || !partial_slots) return fail(WH_EINVAL, "null request or output");
+    Layout L;
+    int rc = build_layout(req, 1, chunk_terms, L);
+    if (rc) return rc;
+    *partial_slots = L.blocks;
     return WH_OK;
 }
 

@@ -20,31 +20,27 @@ import numpy as np
 
 from . import _hafnian as H
 from . import _native
-from .batch import job_elements, pack_jobs
+from .batch import layout_packed, pack_one
 
 __all__ = ["Server", "ServerStats"]
 
 _JOB_BYTES = 256  # upper bound on sizeof(wh_job_dev), rounded
 
 
-def _partial_slots(lib, job, chunk_terms):
-    """Partial-sum slots (term chunks) of one request, as the library lays it out; the layout
-    depends on the request alone, so this is also its share of any batch."""
-    reqs, _ = pack_jobs([job], pool=np.empty(job_elements(job), dtype=np.complex128))
-    b = ctypes.c_void_p()
-    _native.check(lib.wh_batch_create(reqs, 1, chunk_terms, ctypes.byref(b)))
+def _partial_slots(lib, packed, chunk_terms):
+    """Partial-sum slots (term chunks) of one packed request, as the library lays it out; the
+    layout depends on the request alone, so this is also its share of any batch."""
     ps = ctypes.c_int64()
-    _native.check(lib.wh_batch_info(b, None, ctypes.byref(ps), None, None))
-    lib.wh_batch_destroy(b)
+    _native.check(lib.wh_request_partials(ctypes.byref(packed.req), chunk_terms, ctypes.byref(ps)))
     return int(ps.value)
 
 
 class _Item:
-    __slots__ = ("job", "future", "t_submit", "t_launch", "t_done", "elems", "slots", "value", "t_deadline")
+    __slots__ = ("packed", "future", "t_submit", "t_launch", "t_done", "elems", "slots", "value", "t_deadline")
 
-    def __init__(self, job, future, elems, slots, t_submit, t_deadline=None):
-        self.job, self.future = job, future
-        self.elems, self.slots = elems, slots
+    def __init__(self, packed, future, slots, t_submit, t_deadline=None):
+        self.packed, self.future = packed, future
+        self.elems, self.slots = packed.elems, slots
         self.t_submit = t_submit
         self.t_launch = self.t_done = None
         self.value = None
@@ -98,6 +94,12 @@ class _Slot:
         self.items = None
         self.batch = None
         self.footprint = 0
+        self.pool_view = self.h_pool.numpy().view(np.complex128)
+        self.out_view = self.h_out.numpy().view(np.complex128)
+        # raw addresses for the library calls of the serving loop
+        self.ptrs = (self.h_jobs.data_ptr(), self.d_jobs.data_ptr(), self.h_jobs.numel(), self.h_pool.data_ptr(),
+                     self.d_pool.data_ptr(), self.d_partials.data_ptr(), partial_slots, self.d_out.data_ptr(),
+                     self.h_out.data_ptr(), max_requests, self.stream.cuda_stream)
 
 
 class Server:
@@ -129,6 +131,7 @@ class Server:
         self.queue = collections.deque()
         self.admitted_bytes = 0
         self.pending = 0  # submitted and not yet completed
+        self.queued_deadlines = 0  # queued requests with a deadline (the queue is scanned while > 0)
         self.stats = ServerStats()
         self._lock = threading.Lock()
         self._wake = threading.Condition(self._lock)
@@ -156,7 +159,8 @@ class Server:
         if not isinstance(r, H.SieveJob):
             fut.set_result(r)
             return fut
-        item = _Item(r, fut, job_elements(r), _partial_slots(self._lib, r, self.chunk_terms), t_submit,
+        packed = pack_one(r)
+        item = _Item(packed, fut, _partial_slots(self._lib, packed, self.chunk_terms), t_submit,
                      None if deadline_s is None else t_submit + float(deadline_s))
         slot0 = self.slots[0]
         if item.elems > slot0.pool_elems or item.slots > slot0.partial_slots:
@@ -166,6 +170,8 @@ class Server:
         with self._wake:
             self.queue.append(item)
             self.pending += 1
+            if item.t_deadline is not None:
+                self.queued_deadlines += 1
             self._wake.notify()
         return fut
 
@@ -174,28 +180,16 @@ class Server:
 
     # ---------------------------------------------------------------- dispatcher
     def _launch(self, slot, items):
-        torch = self.torch
-        pool = slot.h_pool.numpy().view(np.complex128)
-        jobs = [it.job for it in items]
-        reqs, used = pack_jobs(jobs, pool=pool)
+        reqs, used = layout_packed([it.packed for it in items], slot.pool_view)
         b = ctypes.c_void_p()
-        _native.check(self._lib.wh_batch_create(reqs, len(jobs), self.chunk_terms, ctypes.byref(b)))
-        jb = ctypes.c_size_t()
-        ps = ctypes.c_int64()
-        _native.check(self._lib.wh_batch_info(b, ctypes.byref(jb), ctypes.byref(ps), None, None))
-        if ps.value > slot.partial_slots or jb.value > slot.h_jobs.numel():
+        _native.check(self._lib.wh_batch_create(reqs, len(items), self.chunk_terms, ctypes.byref(b)))
+        # job table, pool copy, kernels and the results' copy back: one library call
+        hj, dj, jcap, hp, dp, dpart, pcap, dout, hout, ocap, strm = slot.ptrs
+        rc = self._lib.wh_batch_submit(b, hj, dj, jcap, hp, dp, used.shape[0], dpart, pcap, dout, hout, ocap, strm)
+        if rc != 0:
             self._lib.wh_batch_destroy(b)
-            raise RuntimeError("batch layout exceeds the slot arena")
-        _native.check(self._lib.wh_batch_write_jobs(b, ctypes.c_void_p(slot.h_jobs.data_ptr())))
-        n2 = 2 * used.shape[0]
-        with torch.cuda.stream(slot.stream):
-            slot.d_pool[:n2].copy_(slot.h_pool[:n2], non_blocking=True)
-            _native.check(self._lib.wh_batch_launch(
-                b, ctypes.c_void_p(slot.h_jobs.data_ptr()), ctypes.c_void_p(slot.d_jobs.data_ptr()),
-                ctypes.c_void_p(slot.d_pool.data_ptr()), ctypes.c_void_p(slot.d_partials.data_ptr()),
-                ctypes.c_void_p(slot.d_out.data_ptr()), ctypes.c_void_p(slot.stream.cuda_stream)))
-            slot.h_out[:2 * len(jobs)].copy_(slot.d_out[:2 * len(jobs)], non_blocking=True)
-            slot.event.record(slot.stream)
+            _native.check(rc)
+        slot.event.record(slot.stream)
         now = time.perf_counter()
         for it in items:
             it.t_launch = now
@@ -205,7 +199,7 @@ class Server:
         self.stats.batch_sizes.append(len(items))
 
     def _complete(self, slot):
-        out = slot.h_out.numpy()[: 2 * len(slot.items)].view(np.complex128).copy()
+        out = slot.out_view[: len(slot.items)].copy()
         now = time.perf_counter()
         self._lib.wh_batch_destroy(slot.batch)
         for k, it in enumerate(slot.items):
@@ -221,8 +215,11 @@ class Server:
         slot.footprint = 0
 
     def _expire(self):
-        """Drops queued requests that were cancelled or whose queue deadline has passed (the
-        caller holds the lock)."""
+        """Drops queued requests whose queue deadline has passed, and cancelled ones on the way
+        (the caller holds the lock).  Without deadlines in the queue there is nothing to scan:
+        a cancelled request is dropped when it reaches the head (``_take``)."""
+        if self.queued_deadlines == 0:
+            return
         now = time.perf_counter()
         keep = collections.deque()
         for it in self.queue:
@@ -235,6 +232,9 @@ class Server:
                 it.future.set_exception(TimeoutError("request not launched within its deadline"))
             else:
                 keep.append(it)
+                continue
+            if it.t_deadline is not None:
+                self.queued_deadlines -= 1
         self.queue = keep
 
     def _take(self, slot):
@@ -247,6 +247,8 @@ class Server:
                     or self.admitted_bytes + fp + f > self.hbm_budget):
                 break
             self.queue.popleft()
+            if it.t_deadline is not None:
+                self.queued_deadlines -= 1
             # RUNNING from here on (no longer cancellable); False: cancelled while queued
             if not it.future.set_running_or_notify_cancel():
                 self.pending -= 1
