@@ -149,6 +149,28 @@ def load(path=None):
         return L
 
 
+_host_lib = None
+
+
+def load_host():
+    """The library's host-only layout calls (no HIP call, microseconds each) through a handle
+    that keeps the interpreter lock across the call: a serving thread that makes them once per
+    request then does not hand the lock to the dispatcher at every request."""
+    global _host_lib
+    load()
+    with _lock:
+        if _host_lib is None:
+            H = ctypes.PyDLL(LIB_PATH)
+            H.wh_request_partials.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, _ip]
+            H.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_void_p)]
+            H.wh_batch_destroy.argtypes = [ctypes.c_void_p]
+            for name in ("wh_request_partials", "wh_batch_create", "wh_batch_destroy"):
+                getattr(H, name).restype = ctypes.c_int
+            _host_lib = H
+        return _host_lib
+
+
 def check(rc):
     """Raise the library's error message for a nonzero return code."""
     if rc != 0:

@@ -226,8 +226,8 @@ size_t wh_generic_lds_bytes(int MM) { return generic_lds_bytes(MM); }
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
                              int64_t nblocks, int64_t block_offset, int MM, hipStream_t stream) {
     size_t lds = wh_generic_lds_bytes(MM);
-    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_generic,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_generic, lds, lim);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(wh_sieve_generic, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
                        njobs, (const cplx*)pool, (double4*)partials, block_offset, MM);

@@ -323,8 +323,8 @@ size_t wh_lhb_lds_bytes(int MM) { return generic_lds_bytes(MM) + (MM + 3 * (WH_N
 hipError_t wh_launch_lhb(const wh_lhb_job* jobs, int njobs, const void* pool, void* partials,
                          int64_t nblocks, int MM, void* out, hipStream_t stream) {
     size_t lds = wh_lhb_lds_bytes(MM);
-    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_lhb,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_lhb, lds, lim);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(wh_sieve_lhb, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs, njobs,
                        (const cplx*)pool, (double4*)partials, (int64_t)0, MM);

@@ -585,8 +585,8 @@ template <int T, bool LOOP>
 static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
     size_t lds = wh_pm1s_lds_bytes(T, nw);
-    hipError_t err = hipFuncSetAttribute((const void*)wh_sieve_pm1s<T, LOOP>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP>, lds, lim);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL((wh_sieve_pm1s<T, LOOP>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream, jobs,
                        njobs, (const cplx*)pool, (double4*)partials, block_offset);

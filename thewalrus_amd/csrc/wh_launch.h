@@ -4,6 +4,29 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <mutex>
+
+// A kernel's dynamic-LDS limit, raised only when a launch needs more than the kernel was last
+// given on the current device: hipFuncSetAttribute costs about as much as the launch itself, and
+// the limit never has to shrink.  One instance per kernel (a function-local static).
+struct wh_lds_limit {
+    std::atomic<size_t> bytes[64];  // per device; zero in static storage
+    std::mutex mu;
+};
+inline hipError_t wh_raise_lds_limit(const void* fn, size_t lds, wh_lds_limit& lim) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lim.bytes[dev].load(std::memory_order_acquire) >= lds) return hipSuccess;
+    std::lock_guard<std::mutex> g(lim.mu);
+    if (lim.bytes[dev].load(std::memory_order_relaxed) >= lds) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) lim.bytes[dev].store(lds, std::memory_order_release);
+    return e;
+}
+
 struct wh_job_dev;
 struct wh_lhb_job;
 

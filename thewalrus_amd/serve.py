@@ -137,6 +137,7 @@ class Server:
         self._wake = threading.Condition(self._lock)
         self._stop = False
         self._lib = _native.load()
+        self._host = _native.load_host()  # host-only layout calls: the interpreter lock stays held
         self._thread = threading.Thread(target=self._run, name="hafnian-dispatcher", daemon=True)
         self._thread.start()
 
@@ -160,7 +161,7 @@ class Server:
             fut.set_result(r)
             return fut
         packed = pack_one(r)
-        item = _Item(packed, fut, _partial_slots(self._lib, packed, self.chunk_terms), t_submit,
+        item = _Item(packed, fut, _partial_slots(self._host, packed, self.chunk_terms), t_submit,
                      None if deadline_s is None else t_submit + float(deadline_s))
         slot0 = self.slots[0]
         if item.elems > slot0.pool_elems or item.slots > slot0.partial_slots:
@@ -182,12 +183,12 @@ class Server:
     def _launch(self, slot, items):
         reqs, used = layout_packed([it.packed for it in items], slot.pool_view)
         b = ctypes.c_void_p()
-        _native.check(self._lib.wh_batch_create(reqs, len(items), self.chunk_terms, ctypes.byref(b)))
+        _native.check(self._host.wh_batch_create(reqs, len(items), self.chunk_terms, ctypes.byref(b)))
         # job table, pool copy, kernels and the results' copy back: one library call
         hj, dj, jcap, hp, dp, dpart, pcap, dout, hout, ocap, strm = slot.ptrs
         rc = self._lib.wh_batch_submit(b, hj, dj, jcap, hp, dp, used.shape[0], dpart, pcap, dout, hout, ocap, strm)
         if rc != 0:
-            self._lib.wh_batch_destroy(b)
+            self._host.wh_batch_destroy(b)
             _native.check(rc)
         slot.event.record(slot.stream)
         now = time.perf_counter()
@@ -201,7 +202,7 @@ class Server:
     def _complete(self, slot):
         out = slot.out_view[: len(slot.items)].copy()
         now = time.perf_counter()
-        self._lib.wh_batch_destroy(slot.batch)
+        self._host.wh_batch_destroy(slot.batch)
         for k, it in enumerate(slot.items):
             it.t_done = now
             self.stats.submit.append(it.t_submit)
