@@ -75,11 +75,13 @@ def serve_run(reqs, qps, Server):
 
 
 def serve_closed(reqs, Server, reps=3):
-    """Every request submitted at once to a warmed server, ``reps`` times: (median hafnians/s
-    from the first submission to the last result, the values of the last pass)."""
+    """Every request submitted at once to a server, after one untimed pass (first launch of
+    each kernel class), ``reps`` times: (median hafnians/s from the first submission to the
+    last result, the values of the last pass)."""
     rates = []
     with Server(slots=4, slot_pool_mb=16) as srv:
-        srv.submit(reqs[0][0], loop=reqs[0][1]).result()
+        for f in [srv.submit(A, loop=l) for A, l in reqs]:
+            f.result()
         for _ in range(reps):
             t0 = time.perf_counter()
             futs = [srv.submit(A, loop=l) for A, l in reqs]

@@ -171,6 +171,49 @@ int wh_request_partials(const wh_request* req, int64_t chunk_terms, int64_t* par
 int wh_batch_chunks(const wh_batch* batch, int64_t* first_chunk, int64_t* nchunks,
                     int64_t* chunk_terms);
 
+/* ---- Serving engine (continuous batching; no counterpart in the reference, whose requests
+ * run one numba call each).  A native dispatcher thread owns `nslots` slots, each with its own
+ * stream and fixed arenas allocated here once (pool_elems complex128, partial_slots partial
+ * sums, max_requests results).  Whenever a slot is free it takes queued requests in arrival
+ * order while they fit the slot and the HBM budget (16 B per pool element + 32 B per partial
+ * slot + 512 B per request, over all launched, unfinished batches), lays them out and enqueues
+ * the copies, kernels and result copy on the slot's stream.  A request's layout depends on the
+ * request alone, so its value is the same alone and in any mix. */
+typedef struct wh_server wh_server;
+
+int wh_server_create(int device, int nslots, int64_t pool_elems, int64_t partial_slots, int64_t max_requests,
+                     int64_t chunk_terms, int64_t hbm_budget_bytes, wh_server** server);
+
+/* Queues one request: `req` with offsets into `payload` (elems complex128, copied here), an
+ * identifying `tag`, and a queue deadline in seconds (< 0: none): still queued past it, the
+ * request completes with status 1 instead.  Fails (nothing queued) for an invalid request or
+ * one that exceeds a slot's arenas or the HBM budget.  Thread-safe. */
+int wh_server_submit(wh_server* server, const wh_request* req, const double* payload, int64_t elems,
+                     uint64_t tag, double deadline_s);
+
+/* Waits up to timeout_s for completions and returns how many were written (at most `max`):
+ * tags, values (2 doubles each), status (0 = done, 1 = deadline passed in the queue, < 0 = the
+ * error code of a failed batch; wh_server_stats gives its message) and times (2 doubles each:
+ * seconds from submission to launch, and from launch to completion).  After the first
+ * completion it waits up to linger_s more while other requests are queued or running, so that
+ * a caller collects completions in groups.  Returns 0 on timeout and -1 once the server has
+ * shut down and every completion was collected. */
+int64_t wh_server_wait(wh_server* server, uint64_t* tags, double* values, int32_t* status, double* times,
+                       int64_t max, double timeout_s, double linger_s);
+
+/* Removes a queued request: 1 if it was still queued (it will not complete), 0 otherwise. */
+int wh_server_cancel(wh_server* server, uint64_t tag);
+
+/* Counters [batches, launched requests, cancelled, expired, queued, in flight] and the last
+ * failure's message (err_bytes bytes at most, NUL-terminated). */
+int wh_server_stats(wh_server* server, int64_t* counters6, char* err, int64_t err_bytes);
+
+/* Stops taking submissions, runs everything queued, and joins the dispatcher thread. */
+int wh_server_shutdown(wh_server* server);
+
+/* Shuts down (if needed) and frees the arenas, streams and the server. */
+int wh_server_destroy(wh_server* server);
+
 int wh_batch_destroy(wh_batch* batch);
 
 /* ---- Plans: a batch plus device buffers owned by the library (convenience form). */

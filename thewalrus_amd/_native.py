@@ -48,6 +48,13 @@ EXPORTS = [
     "wh_batch_launch",
     "wh_batch_submit",
     "wh_request_partials",
+    "wh_server_create",
+    "wh_server_submit",
+    "wh_server_wait",
+    "wh_server_cancel",
+    "wh_server_stats",
+    "wh_server_shutdown",
+    "wh_server_destroy",
     "wh_batch_chunks",
     "wh_batch_destroy",
     "wh_plan_create",
@@ -126,6 +133,15 @@ def load(path=None):
                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p]
         L.wh_request_partials.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, _ip]
+        L.wh_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+        L.wh_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+        L.wh_server_wait.restype = ctypes.c_int64
+        L.wh_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p,
+                                      ctypes.c_int64]
+        L.wh_server_shutdown.argtypes = [ctypes.c_void_p]
+        L.wh_server_destroy.argtypes = [ctypes.c_void_p]
         L.wh_batch_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         L.wh_batch_destroy.argtypes = [ctypes.c_void_p]
         L.wh_plan_partials.argtypes = [ctypes.c_void_p, _dp]
@@ -153,9 +169,10 @@ _host_lib = None
 
 
 def load_host():
-    """The library's host-only layout calls (no HIP call, microseconds each) through a handle
-    that keeps the interpreter lock across the call: a serving thread that makes them once per
-    request then does not hand the lock to the dispatcher at every request."""
+    """The library's short host-only calls (layout, and the serving engine's queue operations:
+    no HIP call, microseconds each) through a handle that keeps the interpreter lock across the
+    call: a thread that makes them once per request then does not hand the lock over at every
+    request."""
     global _host_lib
     load()
     with _lock:
@@ -165,7 +182,11 @@ def load_host():
             H.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(ctypes.c_void_p)]
             H.wh_batch_destroy.argtypes = [ctypes.c_void_p]
-            for name in ("wh_request_partials", "wh_batch_create", "wh_batch_destroy"):
+            H.wh_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(WhRequest), ctypes.c_void_p,
+                                           ctypes.c_int64, ctypes.c_uint64, ctypes.c_double]
+            H.wh_server_cancel.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+            for name in ("wh_request_partials", "wh_batch_create", "wh_batch_destroy", "wh_server_submit",
+                         "wh_server_cancel"):
                 getattr(H, name).restype = ctypes.c_int
             _host_lib = H
         return _host_lib

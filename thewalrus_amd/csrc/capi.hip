@@ -5,12 +5,16 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/walrus_hip.h"
@@ -903,6 +907,389 @@ int wh_hermite_multidimensional(const double* R, const double* y, int64_t n, con
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(G);
     if (e != hipSuccess) return fail(WH_EDEVICE, "hermite recursion: %s", hipGetErrorString(e));
+    return WH_OK;
+}
+
+}  // extern "C"
+
+// ---- Serving engine: a native dispatcher thread over fixed per-slot arenas.  Requests queue
+// with their packed payload; whenever a slot is free the thread takes queued requests in
+// arrival order while they fit the slot and the HBM budget, lays them out, and enqueues copies,
+// kernels and the result copy on the slot's stream; finished slots become completions that
+// callers collect with wh_server_wait.  Layout depends on each request alone, so a result is
+// the same alone and in any mix.
+namespace {
+
+double steady_now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SrvItem {
+    uint64_t tag = 0;
+    wh_request req;
+    std::vector<double> payload;  // 2 * elems doubles
+    int64_t elems = 0, slots = 0, footprint = 0;
+    double t_submit = 0.0, t_deadline = -1.0;
+};
+
+struct SrvDone {
+    uint64_t tag;
+    double re, im;
+    int32_t status;
+    double wait_s, run_s;  // submit -> launch, launch -> completion seen
+};
+
+struct SrvSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t event = nullptr;
+    double* h_pool = nullptr;
+    double* d_pool = nullptr;
+    wh_job_dev* h_jobs = nullptr;
+    wh_job_dev* d_jobs = nullptr;
+    double4* d_partials = nullptr;
+    double* h_out = nullptr;
+    double* d_out = nullptr;
+    std::vector<uint64_t> tags;
+    std::vector<double> t_submit;
+    double t_launch = 0.0;
+    int64_t footprint = 0;
+    bool busy = false;
+};
+
+constexpr int64_t kSrvJobBytes = 256;  // admission's per-request job-table share (bound on the table entry)
+
+}  // namespace
+
+struct wh_server {
+    int device = 0;
+    int64_t pool_elems = 0, partial_slots = 0, max_requests = 0, chunk_terms = 0, hbm_budget = 0;
+    std::vector<SrvSlot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SrvItem> queue;
+    std::deque<SrvDone> done;
+    int64_t admitted = 0;  // footprint of launched, unfinished batches
+    int64_t queued_deadlines = 0;
+    int64_t in_flight = 0;
+    bool stop = false, exited = false;
+    bool idle = false;  // the dispatcher waits for work (submissions wake it only then)
+    int64_t batches = 0, launched = 0, cancelled = 0, expired = 0;
+    std::string error;  // the engine's last failure
+    std::thread th;
+};
+
+namespace {
+
+void srv_free(wh_server* s) {
+    for (SrvSlot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.event) (void)hipEventDestroy(sl.event);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.h_pool) (void)hipHostFree(sl.h_pool);
+        if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
+        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        if (sl.d_pool) (void)hipFree(sl.d_pool);
+        if (sl.d_jobs) (void)hipFree(sl.d_jobs);
+        if (sl.d_partials) (void)hipFree(sl.d_partials);
+        if (sl.d_out) (void)hipFree(sl.d_out);
+    }
+    s->slots.clear();
+}
+
+// Enqueues one batch on a free slot (engine thread, lock not held).  On failure every request of
+// the batch completes with the error status and the slot stays free.
+void srv_launch(wh_server* s, SrvSlot& sl, std::vector<SrvItem>& items, int64_t fp) {
+    const int64_t n = (int64_t)items.size();
+    std::vector<wh_request> reqs(n);
+    int64_t off = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        wh_request r = items[k].req;
+        r.a_off += off;
+        if (r.d_off >= 0) r.d_off += off;
+        if (r.v_off >= 0) r.v_off += off;
+        reqs[k] = r;
+        memcpy(sl.h_pool + 2 * off, items[k].payload.data(), (size_t)items[k].elems * 16);
+        off += items[k].elems;
+    }
+    Layout L;
+    int rc = build_layout(reqs.data(), n, s->chunk_terms, L);
+    if (rc == WH_OK && (L.blocks > s->partial_slots || (int64_t)L.jobs.size() > s->max_requests))
+        rc = fail(WH_EINVAL, "batch exceeds the slot arena");
+    hipError_t e = hipSuccess;
+    if (rc == WH_OK) {
+        memcpy(sl.h_jobs, L.jobs.data(), L.jobs.size() * sizeof(wh_job_dev));
+        e = hipMemcpyAsync(sl.d_pool, sl.h_pool, (size_t)off * 16, hipMemcpyHostToDevice, sl.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, L.jobs.size() * sizeof(wh_job_dev), hipMemcpyHostToDevice,
+                               sl.stream);
+        if (e == hipSuccess) e = run_layout(L, sl.d_jobs, sl.d_pool, sl.d_partials, sl.d_out, sl.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_out, sl.d_out, (size_t)n * 16, hipMemcpyDeviceToHost, sl.stream);
+        if (e == hipSuccess) e = hipEventRecord(sl.event, sl.stream);
+        if (e != hipSuccess) (void)hipStreamSynchronize(sl.stream);  // nothing of the batch still runs
+    }
+    const double now = steady_now();
+    std::lock_guard<std::mutex> g(s->mu);
+    if (rc != WH_OK || e != hipSuccess) {
+        s->error = rc != WH_OK ? g_err : std::string("batch launch failed: ") + hipGetErrorString(e);
+        const int32_t st = rc != WH_OK ? rc : WH_EDEVICE;
+        for (const SrvItem& it : items) s->done.push_back(SrvDone{it.tag, 0.0, 0.0, st, now - it.t_submit, 0.0});
+        s->admitted -= fp;
+        s->in_flight -= n;
+        s->cv_done.notify_all();
+        return;
+    }
+    sl.tags.resize(n);
+    sl.t_submit.resize(n);
+    for (int64_t k = 0; k < n; ++k) {
+        sl.tags[k] = items[k].tag;
+        sl.t_submit[k] = items[k].t_submit;
+    }
+    sl.t_launch = now;
+    sl.footprint = fp;
+    sl.busy = true;
+    s->batches += 1;
+    s->launched += n;
+}
+
+void srv_loop(wh_server* s) {
+    (void)hipSetDevice(s->device);
+    std::vector<SrvItem> batch;
+    for (;;) {
+        bool progressed = false;
+        for (SrvSlot& sl : s->slots) {  // busy flags change on this thread only
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.event);
+            if (q == hipErrorNotReady) continue;
+            const double now = steady_now();
+            std::lock_guard<std::mutex> g(s->mu);
+            const int32_t st = q == hipSuccess ? WH_OK : WH_EDEVICE;
+            if (q != hipSuccess) s->error = std::string("batch failed: ") + hipGetErrorString(q);
+            for (size_t k = 0; k < sl.tags.size(); ++k)
+                s->done.push_back(SrvDone{sl.tags[k], sl.h_out[2 * k], sl.h_out[2 * k + 1], st,
+                                          sl.t_launch - sl.t_submit[k], now - sl.t_launch});
+            s->admitted -= sl.footprint;
+            s->in_flight -= (int64_t)sl.tags.size();
+            sl.busy = false;
+            sl.tags.clear();
+            sl.footprint = 0;
+            progressed = true;
+        }
+        if (progressed) s->cv_done.notify_all();
+
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->queued_deadlines > 0) {  // queued requests past their deadline fail
+            const double now = steady_now();
+            bool any = false;
+            for (auto it = s->queue.begin(); it != s->queue.end();) {
+                if (it->t_deadline >= 0.0 && now > it->t_deadline) {
+                    s->done.push_back(SrvDone{it->tag, 0.0, 0.0, 1, now - it->t_submit, 0.0});
+                    s->queued_deadlines -= 1;
+                    s->expired += 1;
+                    it = s->queue.erase(it);
+                    any = true;
+                } else {
+                    ++it;
+                }
+            }
+            if (any) s->cv_done.notify_all();
+        }
+        bool any_busy = false;
+        SrvSlot* free_slot = nullptr;
+        for (SrvSlot& sl : s->slots) {
+            if (sl.busy) any_busy = true;
+            else if (!free_slot) free_slot = &sl;
+        }
+        if (s->stop && s->queue.empty() && !any_busy) {
+            s->exited = true;
+            lk.unlock();
+            s->cv_done.notify_all();
+            return;
+        }
+        if (free_slot && !s->queue.empty()) {
+            batch.clear();
+            int64_t elems = 0, parts = 0, fp = 0;
+            while (!s->queue.empty() && (int64_t)batch.size() < s->max_requests) {
+                const SrvItem& it = s->queue.front();
+                if (elems + it.elems > s->pool_elems || parts + it.slots > s->partial_slots ||
+                    s->admitted + fp + it.footprint > s->hbm_budget)
+                    break;
+                elems += it.elems;
+                parts += it.slots;
+                fp += it.footprint;
+                if (it.t_deadline >= 0.0) s->queued_deadlines -= 1;
+                batch.push_back(std::move(s->queue.front()));
+                s->queue.pop_front();
+            }
+            if (!batch.empty()) {
+                s->admitted += fp;
+                s->in_flight += (int64_t)batch.size();
+                lk.unlock();
+                srv_launch(s, *free_slot, batch, fp);
+                continue;
+            }
+        }
+        if (!progressed) {
+            s->idle = true;
+            s->cv_work.wait_for(lk, any_busy ? std::chrono::microseconds(20) : std::chrono::milliseconds(2));
+            s->idle = false;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int wh_server_create(int device, int nslots, int64_t pool_elems, int64_t partial_slots, int64_t max_requests,
+                     int64_t chunk_terms, int64_t hbm_budget_bytes, wh_server** out) {
+    if (!out || nslots < 1 || nslots > 64 || pool_elems < 1 || partial_slots < 1 || max_requests < 1 ||
+        chunk_terms < 0 || hbm_budget_bytes < 1)
+        return fail(WH_EINVAL, "server arguments out of range");
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(device));
+    wh_server* s = new wh_server();
+    s->device = device;
+    s->pool_elems = pool_elems;
+    s->partial_slots = partial_slots;
+    s->max_requests = max_requests;
+    s->chunk_terms = chunk_terms;
+    s->hbm_budget = hbm_budget_bytes;
+    s->slots.resize(nslots);
+    hipError_t e = hipSuccess;
+    for (SrvSlot& sl : s->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.event, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_pool, (size_t)pool_elems * 16, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_pool, (size_t)pool_elems * 16);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&sl.h_jobs, (size_t)max_requests * sizeof(wh_job_dev), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_jobs, (size_t)max_requests * sizeof(wh_job_dev));
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_partials, (size_t)partial_slots * sizeof(double4));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_out, (size_t)max_requests * 16, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_out, (size_t)max_requests * 16);
+    }
+    if (e != hipSuccess) {
+        srv_free(s);
+        delete s;
+        (void)hipSetDevice(prev);
+        return fail(WH_ENOMEM, "server arenas: %s", hipGetErrorString(e));
+    }
+    (void)hipSetDevice(prev);
+    s->th = std::thread(srv_loop, s);
+    *out = s;
+    return WH_OK;
+}
+
+int wh_server_submit(wh_server* s, const wh_request* req, const double* payload, int64_t elems, uint64_t tag,
+                     double deadline_s) {
+    if (!s || !req || !payload || elems < 1) return fail(WH_EINVAL, "null server, request or payload");
+    Layout L;
+    int rc = build_layout(req, 1, s->chunk_terms, L);
+    if (rc) return rc;
+    if (job_pool_end(L.jobs[0]) > elems) return fail(WH_EINVAL, "request reads past its payload");
+    if (elems > s->pool_elems || L.blocks > s->partial_slots)
+        return fail(WH_EINVAL, "request exceeds the per-slot arena (%lld elements, %lld slots)", (long long)elems,
+                    (long long)L.blocks);
+    const int64_t fp = 16 * elems + 32 * L.blocks + 2 * kSrvJobBytes;
+    if (fp > s->hbm_budget)
+        return fail(WH_EINVAL, "request footprint %lld B exceeds the HBM budget %lld B", (long long)fp,
+                    (long long)s->hbm_budget);
+    SrvItem it;
+    it.tag = tag;
+    it.req = *req;
+    it.payload.assign(payload, payload + 2 * elems);
+    it.elems = elems;
+    it.slots = L.blocks;
+    it.footprint = fp;
+    it.t_submit = steady_now();
+    it.t_deadline = deadline_s >= 0.0 ? it.t_submit + deadline_s : -1.0;
+    bool wake;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(WH_EINVAL, "server is shut down");
+        if (it.t_deadline >= 0.0) s->queued_deadlines += 1;
+        s->queue.push_back(std::move(it));
+        wake = s->idle;  // a busy dispatcher looks at the queue before it waits again
+    }
+    if (wake) s->cv_work.notify_one();
+    return WH_OK;
+}
+
+int64_t wh_server_wait(wh_server* s, uint64_t* tags, double* values, int32_t* status, double* times, int64_t max,
+                       double timeout_s, double linger_s) {
+    if (!s || !tags || !values || !status || !times || max < 1) return fail(WH_EINVAL, "null server or buffer");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 0.0),
+                        [s] { return !s->done.empty() || s->exited; });
+    if (s->done.empty()) return s->exited ? -1 : 0;
+    if (linger_s > 0 && (int64_t)s->done.size() < max)  // more on the way: collect them together
+        s->cv_done.wait_for(lk, std::chrono::duration<double>(linger_s), [s, max] {
+            return (int64_t)s->done.size() >= max || (s->in_flight == 0 && s->queue.empty()) || s->exited;
+        });
+    int64_t n = 0;
+    while (n < max && !s->done.empty()) {
+        const SrvDone& d = s->done.front();
+        tags[n] = d.tag;
+        values[2 * n] = d.re;
+        values[2 * n + 1] = d.im;
+        status[n] = d.status;
+        times[2 * n] = d.wait_s;
+        times[2 * n + 1] = d.run_s;
+        s->done.pop_front();
+        ++n;
+    }
+    return n;
+}
+
+int wh_server_cancel(wh_server* s, uint64_t tag) {
+    if (!s) return fail(WH_EINVAL, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+        if (it->tag != tag) continue;
+        if (it->t_deadline >= 0.0) s->queued_deadlines -= 1;
+        s->queue.erase(it);
+        s->cancelled += 1;
+        return 1;
+    }
+    return 0;
+}
+
+int wh_server_stats(wh_server* s, int64_t* out6, char* err, int64_t err_bytes) {
+    if (!s || !out6) return fail(WH_EINVAL, "null server or buffer");
+    std::lock_guard<std::mutex> g(s->mu);
+    out6[0] = s->batches;
+    out6[1] = s->launched;
+    out6[2] = s->cancelled;
+    out6[3] = s->expired;
+    out6[4] = (int64_t)s->queue.size();
+    out6[5] = s->in_flight;
+    if (err && err_bytes > 0) snprintf(err, (size_t)err_bytes, "%s", s->error.c_str());
+    return WH_OK;
+}
+
+int wh_server_shutdown(wh_server* s) {
+    if (!s) return WH_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->exited = true;
+    }
+    s->cv_done.notify_all();
+    return WH_OK;
+}
+
+int wh_server_destroy(wh_server* s) {
+    if (!s) return WH_OK;
+    wh_server_shutdown(s);
+    srv_free(s);
+    delete s;
     return WH_OK;
 }
 
