@@ -108,3 +108,24 @@ def test_queue_deadline_and_cancel():
         assert f_ok.result() == tw.hafnian(small[2])  # same request, same kernel and chunking
         assert f_big.result() == tw.hafnian(big, loop=True)
         assert srv.stats.expired == 1 and srv.stats.cancelled == 1
+
+
+def test_close_runs_queued_requests():
+    """close() with requests still queued behind a busy slot: every future gets its value, the
+    counters stay readable, and a later submit is refused."""
+    rng = np.random.default_rng(17)
+    big = random_symmetric(rng, 36, scale=1 / np.sqrt(36))
+    small = [random_symmetric(rng, 2 * int(rng.integers(4, 10)), scale=0.3) for _ in range(40)]
+    srv = Server(slots=1, slot_pool_mb=2)
+    try:
+        f_big = srv.submit(big)
+        futs = [srv.submit(A, loop=bool(k % 2)) for k, A in enumerate(small)]
+    finally:
+        srv.close()
+    assert f_big.done() and all(f.done() for f in futs)
+    assert complex(f_big.result()) == complex(tw.hafnian(big))
+    for k, (A, f) in enumerate(zip(small, futs)):
+        assert complex(f.result()) == complex(tw.hafnian(A, loop=bool(k % 2)))
+    assert srv.stats.launched == 41 and srv.stats.batches >= 2
+    with pytest.raises(RuntimeError):
+        srv.submit(small[0])
