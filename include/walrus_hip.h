@@ -191,6 +191,21 @@ int wh_server_create(int device, int nslots, int64_t pool_elems, int64_t partial
 int wh_server_submit(wh_server* server, const wh_request* req, const double* payload, int64_t elems,
                      uint64_t tag, double deadline_s);
 
+/* wh_server_submit for the common request, prepared here in one pass: A is an n x n row-major
+ * matrix (complex128 when is_complex, else float64), hafnian(A, loop) with n even, 6 <= n <= 64,
+ * A exactly symmetric and NaN-free and |A[0][1]| clearly above atol + rtol |A[0][1]| (so no
+ * validation error or shortcut of thewalrus/_hafnian.py:718-818 applies; glynn is ignored for
+ * loops, as there).  Returns 1 when queued, 0 when the request needs the general host path
+ * (nothing queued), or a negative error. */
+int64_t wh_server_submit_matrix(wh_server* server, const double* A, int64_t n, int is_complex, int loop,
+                                int glynn, double rtol, double atol, uint64_t tag, double deadline_s);
+
+/* The same preparation without a server: writes the request (offsets into `payload`) and the
+ * payload (reordered matrix, then the loop weights), and returns the payload's complex128
+ * element count, or 0 when the request needs the general host path.  Host only. */
+int64_t wh_prepare_allones(const double* A, int64_t n, int is_complex, int loop, int glynn, double rtol,
+                           double atol, wh_request* req, double* payload, int64_t payload_elems);
+
 /* Waits up to timeout_s for completions and returns how many were written (at most `max`):
  * tags, values (2 doubles each), status (0 = done, 1 = deadline passed in the queue, < 0 = the
  * error code of a failed batch; wh_server_stats gives its message) and times (2 doubles each:

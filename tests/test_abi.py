@@ -104,3 +104,58 @@ def test_batch_submit_checks_arenas_before_any_device_call():
         assert rc != 0
     finally:
         L.wh_batch_destroy(b)
+
+
+def _prep_native(L, A, loop, glynn=1, rtol=1e-05, atol=1e-08):
+    import numpy as np
+
+    n = A.shape[0]
+    req = _native.WhRequest()
+    payload = np.zeros(n * n + n, dtype=np.complex128)
+    k = L.wh_prepare_allones(A.ctypes.data, n, int(np.iscomplexobj(A)), int(loop), glynn, rtol, atol,
+                             ctypes.byref(req), payload.ctypes.data, payload.shape[0])
+    return k, req, payload[:k] if k > 0 else None
+
+
+def test_native_preparation_equals_host_path():
+    """wh_prepare_allones (the serving engine's one-pass preparation) writes exactly the request
+    and payload that prepare_hafnian + pack_one produce, for every size and kind it accepts."""
+    import numpy as np
+
+    from thewalrus_amd import _hafnian as H
+    from thewalrus_amd import batch as B
+
+    L = _native.load()
+    rng = np.random.default_rng(11)
+    for n in range(6, 65, 2):
+        for kind in ("complex", "real", "real-as-complex"):
+            X = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if kind == "complex" else 0)
+            A = (X + X.T) / 2
+            if kind == "real-as-complex":
+                A = A.astype(np.complex128)
+            for loop in (False, True):
+                for method in ("glynn", "inclexcl"):
+                    k, req, payload = _prep_native(L, A, loop, glynn=int(method == "glynn"))
+                    want = B.pack_one(H.prepare_hafnian(A, loop=loop, method=method))
+                    assert k == want.elems, (n, kind, loop)
+                    assert np.array_equal(payload.view(np.float64), want.payload.view(np.float64))
+                    assert bytes(req) == bytes(want.req), (n, kind, loop, method)
+
+
+def test_native_preparation_declines_other_requests():
+    """Everything outside the one-pass case goes back to the host path (return 0)."""
+    import numpy as np
+
+    L = _native.load()
+    rng = np.random.default_rng(12)
+    X = rng.normal(size=(8, 8))
+    S = (X + X.T) / 2
+    near = S.copy()
+    near[2, 5] += 1e-12  # symmetric within tolerance, not exactly
+    nan = S.copy()
+    nan[3, 3] = np.nan
+    diag = np.diag(rng.normal(size=8)) + 1e-9 * np.ones((8, 8))
+    for A, loop in ((near, False), (nan, True), (diag, False), (S[:4, :4].copy(), False), (S[:7, :7].copy(), True),
+                    (np.zeros((66, 66)), False)):
+        k, _, _ = _prep_native(L, np.ascontiguousarray(A), loop)
+        assert k == 0

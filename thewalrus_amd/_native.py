@@ -50,6 +50,8 @@ EXPORTS = [
     "wh_request_partials",
     "wh_server_create",
     "wh_server_submit",
+    "wh_server_submit_matrix",
+    "wh_prepare_allones",
     "wh_server_wait",
     "wh_server_cancel",
     "wh_server_stats",
@@ -140,6 +142,10 @@ def load(path=None):
         L.wh_server_wait.restype = ctypes.c_int64
         L.wh_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p,
                                       ctypes.c_int64]
+        L.wh_prepare_allones.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.POINTER(WhRequest),
+                                         ctypes.c_void_p, ctypes.c_int64]
+        L.wh_prepare_allones.restype = ctypes.c_int64
         L.wh_server_shutdown.argtypes = [ctypes.c_void_p]
         L.wh_server_destroy.argtypes = [ctypes.c_void_p]
         L.wh_batch_chunks.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
@@ -185,9 +191,13 @@ def load_host():
             H.wh_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(WhRequest), ctypes.c_void_p,
                                            ctypes.c_int64, ctypes.c_uint64, ctypes.c_double]
             H.wh_server_cancel.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+            H.wh_server_submit_matrix.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_uint64, ctypes.c_double]
             for name in ("wh_request_partials", "wh_batch_create", "wh_batch_destroy", "wh_server_submit",
                          "wh_server_cancel"):
                 getattr(H, name).restype = ctypes.c_int
+            H.wh_server_submit_matrix.restype = ctypes.c_int64
             _host_lib = H
         return _host_lib
 

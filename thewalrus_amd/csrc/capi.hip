@@ -912,6 +912,71 @@ int wh_hermite_multidimensional(const double* R, const double* y, int64_t n, con
 
 }  // extern "C"
 
+// ---- Host preparation of the common request in one pass: n modes with one photon each (n even,
+// 6 <= n <= 64), an exactly symmetric NaN-free matrix whose entry (0, 1) is far from zero.  It is
+// what prepare_hafnian / _prepare_haf / _prepare_loop produce for such a matrix
+// (thewalrus/_hafnian.py:718-818, :470-508, :581-631): no validation error is possible (every
+// a_ij equals a_ji, so numpy.allclose(A, A.T) holds under any tolerance), no shortcut applies
+// (n > 4, and |a_01| exceeds atol + rtol |a_01| by a margin wider than any rounding difference,
+// so numpy.allclose(diag(diag(A)), A) is false), and matched_reps pairs mode n-1-2k with
+// n-2-2k.  Anything else returns 0: the caller runs the general host path.
+namespace {
+
+int64_t prep_allones(const double* A, int64_t n, int is_complex, int loop, int glynn, double rtol, double atol,
+                     wh_request& r, double* payload) {
+    if (n < 6 || n > 2 * WH_MAX_EDGES || (n & 1)) return 0;
+    const int64_t st = is_complex ? 2 : 1;
+    auto re = [&](int64_t i, int64_t j) { return A[(i * n + j) * st]; };
+    auto im = [&](int64_t i, int64_t j) { return is_complex ? A[(i * n + j) * 2 + 1] : 0.0; };
+    for (int64_t i = 0; i < n; ++i)  // NaN never compares equal: the diagonal is checked too
+        for (int64_t j = i; j < n; ++j)
+            if (!(re(i, j) == re(j, i)) || !(im(i, j) == im(j, i))) return 0;
+    const double ay = std::hypot(re(0, 1), im(0, 1));
+    if (!(ay < INFINITY) || !(ay * (1.0 - 1e-9) > atol + rtol * ay * (1.0 + 1e-9))) return 0;
+    const int64_t E = n / 2;
+    int64_t x[2 * WH_MAX_EDGES];
+    for (int64_t k = 0; k < E; ++k) {
+        x[k] = n - 1 - 2 * k;
+        x[E + k] = n - 2 - 2 * k;
+    }
+    bool real = true;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) {
+            const double a = re(x[i], x[j]), b = im(x[i], x[j]);
+            payload[2 * (i * n + j)] = a;
+            payload[2 * (i * n + j) + 1] = b;
+            if (b != 0.0) real = false;
+        }
+    int64_t elems = n * n;
+    if (loop) {
+        for (int64_t i = 0; i < n; ++i) {
+            payload[2 * (elems + i)] = re(x[i], x[i]);
+            payload[2 * (elems + i) + 1] = im(x[i], x[i]);
+        }
+        elems += n;
+    }
+    memset(&r, 0, sizeof r);
+    r.a_off = 0;
+    r.d_off = loop ? n * n : -1;
+    r.v_off = -1;
+    r.E = (int32_t)E;
+    r.glynn = loop ? 1 : (glynn != 0);
+    r.real = real ? 1 : 0;
+    r.options = 0;
+    for (int64_t k = 0; k < E; ++k) r.edge_reps[k] = 1;
+    return elems;
+}
+
+}  // namespace
+
+extern "C" int64_t wh_prepare_allones(const double* A, int64_t n, int is_complex, int loop, int glynn, double rtol,
+                                      double atol, wh_request* req, double* payload, int64_t payload_elems) {
+    if (!A || !req || !payload) return fail(WH_EINVAL, "null matrix, request or payload");
+    if (payload_elems < n * n + n) return fail(WH_EINVAL, "payload holds %lld elements, %lld needed",
+                                               (long long)payload_elems, (long long)(n * n + n));
+    return prep_allones(A, n, is_complex, loop, glynn, rtol, atol, *req, payload);
+}
+
 // ---- Serving engine: a native dispatcher thread over fixed per-slot arenas.  Requests queue
 // with their packed payload; whenever a slot is free the thread takes queued requests in
 // arrival order while they fit the slot and the HBM budget, lays them out, and enqueues copies,
@@ -1052,6 +1117,9 @@ void srv_launch(wh_server* s, SrvSlot& sl, std::vector<SrvItem>& items, int64_t 
     s->launched += n;
 }
 
+int srv_enqueue(wh_server* s, const wh_request& req, std::vector<double>&& payload, int64_t elems, uint64_t tag,
+                double deadline_s);
+
 void srv_loop(wh_server* s) {
     (void)hipSetDevice(s->device);
     std::vector<SrvItem> batch;
@@ -1185,8 +1253,17 @@ int wh_server_create(int device, int nslots, int64_t pool_elems, int64_t partial
 int wh_server_submit(wh_server* s, const wh_request* req, const double* payload, int64_t elems, uint64_t tag,
                      double deadline_s) {
     if (!s || !req || !payload || elems < 1) return fail(WH_EINVAL, "null server, request or payload");
+    return srv_enqueue(s, *req, std::vector<double>(payload, payload + 2 * elems), elems, tag, deadline_s);
+}
+
+}  // extern "C"
+
+namespace {
+
+int srv_enqueue(wh_server* s, const wh_request& req, std::vector<double>&& payload, int64_t elems, uint64_t tag,
+                double deadline_s) {
     Layout L;
-    int rc = build_layout(req, 1, s->chunk_terms, L);
+    int rc = build_layout(&req, 1, s->chunk_terms, L);
     if (rc) return rc;
     if (job_pool_end(L.jobs[0]) > elems) return fail(WH_EINVAL, "request reads past its payload");
     if (elems > s->pool_elems || L.blocks > s->partial_slots)
@@ -1198,8 +1275,8 @@ int wh_server_submit(wh_server* s, const wh_request* req, const double* payload,
                     (long long)s->hbm_budget);
     SrvItem it;
     it.tag = tag;
-    it.req = *req;
-    it.payload.assign(payload, payload + 2 * elems);
+    it.req = req;
+    it.payload = std::move(payload);
     it.elems = elems;
     it.slots = L.blocks;
     it.footprint = fp;
@@ -1215,6 +1292,23 @@ int wh_server_submit(wh_server* s, const wh_request* req, const double* payload,
     }
     if (wake) s->cv_work.notify_one();
     return WH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t wh_server_submit_matrix(wh_server* s, const double* A, int64_t n, int is_complex, int loop, int glynn,
+                                double rtol, double atol, uint64_t tag, double deadline_s) {
+    if (!s || !A) return fail(WH_EINVAL, "null server or matrix");
+    if (n < 6 || n > 2 * WH_MAX_EDGES) return 0;
+    std::vector<double> payload((size_t)2 * (n * n + n));
+    wh_request r;
+    const int64_t elems = prep_allones(A, n, is_complex, loop, glynn, rtol, atol, r, payload.data());
+    if (elems == 0) return 0;
+    payload.resize((size_t)2 * elems);
+    const int rc = srv_enqueue(s, r, std::move(payload), elems, tag, deadline_s);
+    return rc == WH_OK ? 1 : rc;
 }
 
 int64_t wh_server_wait(wh_server* s, uint64_t* tags, double* values, int32_t* status, double* times, int64_t max,

@@ -95,6 +95,17 @@ class _ServedFuture(Future):
         return super().cancel()
 
 
+_ONE_PASS_DTYPES = (np.dtype(np.float64), np.dtype(np.complex128))
+
+
+def _one_pass(A, method, rtol, atol):
+    """Whether the library's one-pass preparation may be tried (it declines, returning 0, what
+    it cannot decide exactly as the host path)."""
+    return (type(A) is np.ndarray and A.ndim == 2 and 6 <= A.shape[0] <= 64 and A.shape[0] == A.shape[1]
+            and A.shape[0] % 2 == 0 and A.dtype in _ONE_PASS_DTYPES and A.flags.c_contiguous
+            and method in ("glynn", "inclexcl") and isinstance(rtol, (float, int)) and isinstance(atol, (float, int)))
+
+
 def _stop_engine(lib, srv, collector):
     """Runs what is queued, stops the engine and the completion thread, frees the engine (at
     ``close()``, or at interpreter exit for a server never closed)."""
@@ -150,6 +161,25 @@ class Server:
         with ``TimeoutError`` instead; a launched request always completes.  A request whose
         future is cancelled (``Future.cancel()``) before launch is dropped from the queue."""
         t_submit = time.perf_counter()
+        if rpt is None and _one_pass(A, method, rtol, atol):
+            # the common request (n even, 6..64, exactly symmetric): validated, paired, reordered
+            # and queued by the library in one call; 0 = it needs the host path below
+            tag = next(self._tags)
+            fut = _ServedFuture(self, tag)
+            with self._lock:
+                if self._closed:
+                    raise RuntimeError("server is closed")
+                t_enq = time.perf_counter()
+                rc = self._host.wh_server_submit_matrix(
+                    self._srv, A.ctypes.data, A.shape[0], A.dtype == np.complex128, bool(loop),
+                    method == "glynn", rtol, atol, tag, -1.0 if deadline_s is None else float(deadline_s))
+                if rc == 1:
+                    self._futures[tag] = (fut, t_enq, t_submit)
+                    self.pending += 1
+            if rc == 1:
+                return fut
+            if rc < 0:
+                _native.check(int(rc))
         if rpt is None:
             r = H.prepare_hafnian(A, loop=loop, rtol=rtol, atol=atol, method=method)
         else:
