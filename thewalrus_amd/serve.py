@@ -171,8 +171,9 @@ class Server:
                     raise RuntimeError("server is closed")
                 t_enq = time.perf_counter()
                 rc = self._host.wh_server_submit_matrix(
-                    self._srv, A.ctypes.data, A.shape[0], A.dtype == np.complex128, bool(loop),
-                    method == "glynn", rtol, atol, tag, -1.0 if deadline_s is None else float(deadline_s))
+                    self._srv, A.ctypes.data, A.shape[0], int(A.dtype == np.complex128), int(bool(loop)),
+                    int(method == "glynn"), float(rtol), float(atol), tag,
+                    -1.0 if deadline_s is None else float(deadline_s))
                 if rc == 1:
                     self._futures[tag] = (fut, t_enq, t_submit)
                     self.pending += 1
