@@ -142,6 +142,15 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
     for _ in range(5):
         tw.hafnian(A1)
     out["config1_n20_per_call_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+    if with_cpu:
+        from oracle import oracle as O
+
+        cores = len(os.sched_getaffinity(0))
+        O.hafnian_sieve(A1, nthreads=cores)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            O.hafnian_sieve(A1, nthreads=cores)
+        out["config1_n20_cpu_oracle_ms"] = (time.perf_counter() - t0) / 5 * 1e3
 
     mats = W.config2()
     b2 = B.DeviceBatch([B.prepare(M) for M in mats])
