@@ -210,6 +210,32 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
     ev, wall = timed_runs(b4, 3, torch, stream)
     out["config4_repeated_batch"] = {"requests_per_s": 3 * len(reqs4) / wall, "kernel_ms": ev / 3 * 1e3}
     b4.close()
+    if with_cpu:
+        # the same requests on the CPU oracle: over terms (all threads per request) on 128 of them,
+        # over requests (one single-threaded sieve per thread) on all; the faster kept
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as O
+
+        cores = len(os.sched_getaffinity(0))
+
+        def one(r, nt):
+            A, rpt, mu = r
+            return O.hafnian_sieve(A, loop=True, rpt=rpt, mu=mu, nthreads=nt)
+
+        t0 = time.perf_counter()
+        for r in reqs4[:128]:
+            one(r, cores)
+        rate_terms = 128 / (time.perf_counter() - t0)
+        with ThreadPoolExecutor(cores) as ex:
+            t0 = time.perf_counter()
+            list(ex.map(lambda r: one(r, 1), reqs4))
+            rate_reqs = len(reqs4) / (time.perf_counter() - t0)
+        row = out["config4_repeated_batch"]
+        row["cpu_oracle_requests_per_s"] = max(rate_terms, rate_reqs)
+        row["cpu_oracle_over_terms"] = rate_terms
+        row["cpu_oracle_over_requests"] = rate_reqs
+        row["gpu_over_cpu"] = row["requests_per_s"] / max(rate_terms, rate_reqs)
     return out
 
 
