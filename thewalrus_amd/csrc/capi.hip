@@ -302,6 +302,7 @@ struct Group {
     int64_t block_begin = 0, block_end = 0;
     int nw = 1;
     int MM = 0;  // generic: LDS matrix side
+    int NQ = 0;  // generic: largest photon number + 2 (LDS coefficient rows)
 };
 
 // One past the last pool element a job reads: the 2E x 2E matrix and the 2E-vectors D and V.
@@ -359,7 +360,10 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
         base += j.nchunks;
         gr.job_end = k + 1;
         gr.block_end = base;
-        if (g.kind == KIND_GENERIC) gr.MM = std::max(gr.MM, 2 * j.E);
+        if (g.kind == KIND_GENERIC) {
+            gr.MM = std::max(gr.MM, 2 * j.E);
+            gr.NQ = std::max(gr.NQ, j.N + 2);
+        }
         L.terms += j.steps;
         L.flops += (double)j.steps * term_flops(j, g);
         L.jobs[k] = j;
@@ -386,7 +390,7 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
                               partials, nb,
                               g.block_begin, g.nw, s);
         else
-            e = wh_launch_generic(gj, nj, pool, partials, nb, g.block_begin, g.MM, s);
+            e = wh_launch_generic(gj, nj, pool, partials, nb, g.block_begin, g.MM, g.NQ, s);
         if (e != hipSuccess) return e;
     }
     return wh_launch_fold(d_jobs, (int)L.jobs.size(), partials, out, s);

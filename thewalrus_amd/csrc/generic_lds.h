@@ -18,18 +18,23 @@ struct GenLds {
     cplx* pt;    // power traces p[0..h]
     cplx* lv;    // loop scalars, index by order
     cplx* ov;    // odd-loop scalars
-    cplx* comb;  // 2 x (WH_NMAX + 2) coefficient rows
+    cplx* comb;  // 2 coefficient rows of ncomb entries
+    int ncomb;   // coefficient row length (orders 0 .. N need N + 1 <= ncomb)
     int* S;      // kept edge indices
     int* dS;     // their delta values
 };
 
+// NQ = (largest photon number N of the workgroup's requests) + 2: power traces and loop
+// scalars need indices 0 .. N / 2 + 1, coefficient rows 0 .. N.  The default covers every N.
+__host__ __device__ constexpr size_t generic_pts(int NQ) { return (size_t)(NQ - 2) / 2 + 4; }
+
 // Bytes of the layout carve() lays out (a multiple of 16).
-__host__ __device__ constexpr size_t generic_lds_bytes(int MM) {
-    return (3 * (size_t)MM * MM + 5 * (size_t)MM + 3 * (WH_NMAX / 2 + 4) + 2 * (WH_NMAX + 2)) * sizeof(cplx) +
+__host__ __device__ constexpr size_t generic_lds_bytes(int MM, int NQ = WH_NMAX + 2) {
+    return (3 * (size_t)MM * MM + 5 * (size_t)MM + 3 * generic_pts(NQ) + 2 * (size_t)NQ) * sizeof(cplx) +
            2 * WH_EMAX * sizeof(int);
 }
 
-__device__ __forceinline__ GenLds carve(unsigned char* base, int MM) {
+__device__ __forceinline__ GenLds carve(unsigned char* base, int MM, int NQ = WH_NMAX + 2) {
     GenLds L;
     cplx* c = reinterpret_cast<cplx*>(base);
     size_t mm = (size_t)MM * MM;
@@ -41,10 +46,11 @@ __device__ __forceinline__ GenLds carve(unsigned char* base, int MM) {
     L.OV = c; c += MM;
     L.v = c; c += MM;
     L.w = c; c += MM;
-    L.pt = c; c += WH_NMAX / 2 + 4;
-    L.lv = c; c += WH_NMAX / 2 + 4;
-    L.ov = c; c += WH_NMAX / 2 + 4;
-    L.comb = c; c += 2 * (WH_NMAX + 2);
+    L.pt = c; c += generic_pts(NQ);
+    L.lv = c; c += generic_pts(NQ);
+    L.ov = c; c += generic_pts(NQ);
+    L.comb = c; c += 2 * (size_t)NQ;
+    L.ncomb = NQ;
     L.S = reinterpret_cast<int*>(c);
     L.dS = L.S + WH_EMAX;
     return L;

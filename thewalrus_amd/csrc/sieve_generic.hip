@@ -10,7 +10,7 @@ using namespace wh_generic;
 // partials[b] = (sum_re, err_re, sum_im, err_im) of the chunk's signed terms.
 extern "C" __global__ void __launch_bounds__(WH_WAVE)
 wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
-                 double4* __restrict__ partials, int64_t block_offset, int MM) {
+                 double4* __restrict__ partials, int64_t block_offset, int MM, int NQ) {
     extern __shared__ __align__(16) unsigned char wh_smem[];
     const int lane = threadIdx.x;
     const int64_t b = blockIdx.x + block_offset;
@@ -32,7 +32,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
     const cplx* Dg = loop ? pool + J.d_off : nullptr;
     const cplx* Vg = odd ? pool + J.v_off : nullptr;
     const cplx oddloop = cmk(J.oddloop_re, J.oddloop_im);
-    GenLds L = carve(wh_smem, MM);
+    GenLds L = carve(wh_smem, MM, NQ);
 
     // per-lane constants for the mixed-radix decode (lane i < E owns digit i)
     int my_e = lane < E ? J.e[lane] : 0;
@@ -158,7 +158,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
 
         // _hafnian.py:196-208 (f), :228-241 (f_loop), :264-283 (f_loop_odd)
         cplx* oldc = L.comb;
-        cplx* newc = L.comb + (WH_NMAX + 2);
+        cplx* newc = L.comb + L.ncomb;
         for (int kk = lane; kk <= top; kk += WH_WAVE) oldc[kk] = (kk == 0) ? cmk(1.0, 0.0) : cmk(0.0, 0.0);
         __syncthreads();
         for (int i = 1; i <= top; ++i) {
@@ -221,16 +221,17 @@ extern "C" __global__ void wh_fold(const wh_job_dev* __restrict__ jobs, int njob
 // ---- host-side launchers (kernels are launched from the translation unit that defines them)
 #include "wh_launch.h"
 
-size_t wh_generic_lds_bytes(int MM) { return generic_lds_bytes(MM); }
+size_t wh_generic_lds_bytes(int MM, int NQ) { return generic_lds_bytes(MM, NQ); }
 
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                             int64_t nblocks, int64_t block_offset, int MM, hipStream_t stream) {
-    size_t lds = wh_generic_lds_bytes(MM);
+                             int64_t nblocks, int64_t block_offset, int MM, int NQ, hipStream_t stream) {
+    if (NQ < 4 || NQ > WH_NMAX + 2) return hipErrorInvalidValue;
+    size_t lds = wh_generic_lds_bytes(MM, NQ);
     static wh_lds_limit lim;
     hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_generic, lds, lim);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(wh_sieve_generic, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
-                       njobs, (const cplx*)pool, (double4*)partials, block_offset, MM);
+                       njobs, (const cplx*)pool, (double4*)partials, block_offset, MM, NQ);
     return hipGetLastError();
 }
 

@@ -30,9 +30,9 @@ inline hipError_t wh_raise_lds_limit(const void* fn, size_t lds, wh_lds_limit& l
 struct wh_job_dev;
 struct wh_lhb_job;
 
-size_t wh_generic_lds_bytes(int MM);
+size_t wh_generic_lds_bytes(int MM, int NQ);
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                             int64_t nblocks, int64_t block_offset, int MM, hipStream_t stream);
+                             int64_t nblocks, int64_t block_offset, int MM, int NQ, hipStream_t stream);
 size_t wh_pm1_lds_bytes(int T, bool real, int nw);
 hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_dev* jobs, int njobs,
                          const void* pool, void* partials, int64_t nblocks, int64_t block_offset,
