@@ -28,19 +28,24 @@ struct GenLds {
 // scalars need indices 0 .. N / 2 + 1, coefficient rows 0 .. N.  The default covers every N.
 __host__ __device__ constexpr size_t generic_pts(int NQ) { return (size_t)(NQ - 2) / 2 + 4; }
 
+// Sides up to this form the powers in place (matmul_inplace_lds) and carve no Q.
+constexpr int kGenericInplaceSide = 16;
+
 // Bytes of the layout carve() lays out (a multiple of 16).
-__host__ __device__ constexpr size_t generic_lds_bytes(int MM, int NQ = WH_NMAX + 2) {
-    return (3 * (size_t)MM * MM + 5 * (size_t)MM + 3 * generic_pts(NQ) + 2 * (size_t)NQ) * sizeof(cplx) +
+__host__ __device__ constexpr size_t generic_lds_bytes(int MM, int NQ = WH_NMAX + 2, bool with_q = true) {
+    return ((with_q ? 3 : 2) * (size_t)MM * MM + 5 * (size_t)MM + 3 * generic_pts(NQ) + 2 * (size_t)NQ) *
+               sizeof(cplx) +
            2 * WH_EMAX * sizeof(int);
 }
 
-__device__ __forceinline__ GenLds carve(unsigned char* base, int MM, int NQ = WH_NMAX + 2) {
+__device__ __forceinline__ GenLds carve(unsigned char* base, int MM, int NQ = WH_NMAX + 2, bool with_q = true) {
     GenLds L;
     cplx* c = reinterpret_cast<cplx*>(base);
     size_t mm = (size_t)MM * MM;
     L.H = c; c += mm;
     L.P = c; c += mm;
-    L.Q = c; c += mm;
+    L.Q = nullptr;
+    if (with_q) { L.Q = c; c += mm; }
     L.XD = c; c += MM;
     L.D = c; c += MM;
     L.OV = c; c += MM;
@@ -63,6 +68,31 @@ __device__ void matmul_lds(const cplx* P, const cplx* H, cplx* Q, int m, int MM,
         cplx s = cmk(0.0, 0.0);
         for (int k = 0; k < m; ++k) cmac(s, P[i * MM + k], H[k * MM + j]);
         Q[i * MM + j] = s;
+    }
+}
+
+// P = cur @ H for m <= kGenericInplaceSide (cur may be P): each lane keeps its outputs (at most
+// 4) in registers until every lane has read cur; k summed in order, as matmul_lds.
+__device__ void matmul_inplace_lds(const cplx* cur, const cplx* H, cplx* P, int m, int MM, int lane) {
+    cplx o[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = lane + u * WH_WAVE;
+        cplx s = cmk(0.0, 0.0);
+        if (e < m * m) {
+            const int i = e / m, j = e - i * m;
+            for (int k = 0; k < m; ++k) cmac(s, cur[i * MM + k], H[k * MM + j]);
+        }
+        o[u] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = lane + u * WH_WAVE;
+        if (e < m * m) {
+            const int i = e / m, j = e - i * m;
+            P[i * MM + j] = o[u];
+        }
     }
 }
 

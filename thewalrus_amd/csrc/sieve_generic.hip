@@ -8,7 +8,10 @@ using namespace wh_generic;
 
 // jobs[]: sorted by chunk_base; block b finds its job by binary search.
 // partials[b] = (sum_re, err_re, sum_im, err_im) of the chunk's signed terms.
-extern "C" __global__ void __launch_bounds__(WH_WAVE)
+// INPLACE: every matrix side is at most kGenericInplaceSide, and the powers are formed in place
+// (two m x m matrices in LDS instead of three: more workgroups per CU).
+template <bool INPLACE>
+__global__ void __launch_bounds__(WH_WAVE)
 wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
                  double4* __restrict__ partials, int64_t block_offset, int MM, int NQ) {
     extern __shared__ __align__(16) unsigned char wh_smem[];
@@ -32,7 +35,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
     const cplx* Dg = loop ? pool + J.d_off : nullptr;
     const cplx* Vg = odd ? pool + J.v_off : nullptr;
     const cplx oddloop = cmk(J.oddloop_re, J.oddloop_im);
-    GenLds L = carve(wh_smem, MM, NQ);
+    GenLds L = carve(wh_smem, MM, NQ, !INPLACE);
 
     // per-lane constants for the mixed-radix decode (lane i < E owns digit i)
     int my_e = lane < E ? J.e[lane] : 0;
@@ -132,8 +135,9 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
             if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
             const cplx* cur = L.H;
             for (int it = 0; it < min_val - 2; ++it) {
-                cplx* dst = (cur == L.P) ? L.Q : L.P;
-                matmul_lds(cur, L.H, dst, m, MM, lane);
+                cplx* dst = (INPLACE || cur != L.P) ? L.P : L.Q;
+                if (INPLACE) matmul_inplace_lds(cur, L.H, dst, m, MM, lane);
+                else matmul_lds(cur, L.H, dst, m, MM, lane);
                 __syncthreads();
                 cplx tk = trace_lds(dst, m, MM, lane);
                 if (lane == 0) L.pt[it + 2] = tk;
@@ -221,18 +225,26 @@ extern "C" __global__ void wh_fold(const wh_job_dev* __restrict__ jobs, int njob
 // ---- host-side launchers (kernels are launched from the translation unit that defines them)
 #include "wh_launch.h"
 
-size_t wh_generic_lds_bytes(int MM, int NQ) { return generic_lds_bytes(MM, NQ); }
+size_t wh_generic_lds_bytes(int MM, int NQ) { return generic_lds_bytes(MM, NQ, MM > kGenericInplaceSide); }
+
+template <bool INPLACE>
+static hipError_t launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                                 int64_t nblocks, int64_t block_offset, int MM, int NQ, hipStream_t stream) {
+    size_t lds = generic_lds_bytes(MM, NQ, !INPLACE);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_generic<INPLACE>, lds, lim);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((wh_sieve_generic<INPLACE>), dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
+                       njobs, (const cplx*)pool, (double4*)partials, block_offset, MM, NQ);
+    return hipGetLastError();
+}
 
 hipError_t wh_launch_generic(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
                              int64_t nblocks, int64_t block_offset, int MM, int NQ, hipStream_t stream) {
-    if (NQ < 4 || NQ > WH_NMAX + 2) return hipErrorInvalidValue;
-    size_t lds = wh_generic_lds_bytes(MM, NQ);
-    static wh_lds_limit lim;
-    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_generic, lds, lim);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(wh_sieve_generic, dim3((unsigned)nblocks), dim3(WH_WAVE), lds, stream, jobs,
-                       njobs, (const cplx*)pool, (double4*)partials, block_offset, MM, NQ);
-    return hipGetLastError();
+    if (NQ < 4 || NQ > WH_NMAX + 2 || MM < 1) return hipErrorInvalidValue;
+    if (MM <= kGenericInplaceSide)
+        return launch_generic<true>(jobs, njobs, pool, partials, nblocks, block_offset, MM, NQ, stream);
+    return launch_generic<false>(jobs, njobs, pool, partials, nblocks, block_offset, MM, NQ, stream);
 }
 
 hipError_t wh_launch_fold(const wh_job_dev* jobs, int njobs, const void* partials, void* out,
