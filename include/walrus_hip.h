@@ -120,6 +120,12 @@ typedef struct wh_request {
 /* wh_request.options: on the matrix cores, form the full powers B^a = B^(a-1) B for hafnians
  * instead of the symmetric powers C_a = B^a X diag(d) (upper triangle only, the default). */
 #define WH_OPT_NONSYM 16
+/* wh_request.options: on the general kernel, when h + 1 exceeds the term matrix's side m, take
+ * the traces of B^m .. B^h as the reference does (Householder reduction, La Budde's
+ * characteristic polynomial and the Cayley-Hamilton recurrence, charpoly.py:301-327) instead of
+ * the default, forming those powers directly (the same traces; on the GPU the serial
+ * reduction costs more than the few extra products). */
+#define WH_OPT_CHARPOLY 32
 /* wh_request.options: a montrealer request (montrealer / lmontrealer, _montrealer.py:37-101)
  * over n = E modes.  a_off: X Sigma (2E x 2E, rows of Sigma swapped by halves); for the loop
  * montrealer d_off: X zeta and v_off: conj(X zeta), else both -1.  edge_reps, glynn and

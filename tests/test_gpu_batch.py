@@ -263,3 +263,35 @@ def test_odd_chunk_sizes_agree(loop):
             db.close()
             for k in range(len(reqs)):
                 assert relerr(r[k], r0[k]) < 1e-12, (kw, ct, reqs[k][0].shape[0], relerr(r[k], r0[k]))
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_direct_powers_and_charpoly_routes(loop):
+    """Repeated-row requests whose term matrices are smaller than h + 1 (so the reference extends
+    the power traces by the characteristic polynomial): the general kernel's default (every
+    power formed directly) and the reference's route (WH_OPT_CHARPOLY) both against truth; bar:
+    1e-10 or twice the reference algorithm's own FP64 error on the input.  Odd photon numbers
+    included for loops."""
+    rng = np.random.default_rng(77 + loop)
+    reqs = []
+    for k in range(12):
+        n = int(rng.integers(3, 7))
+        A = random_symmetric(rng, n, scale=1 / np.sqrt(n))
+        rpt = [int(r) for r in rng.integers(1, 6, size=n)]
+        if not loop and sum(rpt) % 2:
+            rpt[0] += 1
+        mu = (rng.normal(size=n) + 1j * rng.normal(size=n)) if loop else None
+        reqs.append((A, rpt, mu))
+    preps = [B.prepare_repeated(A, rpt, mu=mu, loop=loop) for A, rpt, mu in reqs]
+    direct = B.DeviceBatch(preps)
+    charpoly = B.DeviceBatch(preps, charpoly=True)
+    direct.run()
+    charpoly.run()
+    rd, rc = direct.results(), charpoly.results()
+    assert direct.flops > charpoly.flops  # the direct route forms more (small) products
+    for (A, rpt, mu), d, c in zip(reqs, rd, rc):
+        truth = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, precision="dd")
+        ref64 = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, precision="double")
+        bar = max(TOL, 2 * relerr(ref64, truth))
+        assert relerr(d, truth) < bar, ("direct", rpt, relerr(d, truth), bar)
+        assert relerr(c, truth) < bar, ("charpoly", rpt, relerr(c, truth), bar)

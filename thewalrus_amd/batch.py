@@ -153,10 +153,13 @@ class DeviceBatch:
             the vector FMA units instead of the FP64 matrix cores
         full_powers (bool): on the matrix cores, form the full powers B^a instead of the
             symmetric powers B^a X diag(d) (upper triangle only)
+        charpoly (bool): on the general kernel, take the traces above the term matrix's side
+            from the characteristic polynomial as the reference does, instead of forming those
+            powers directly
     """
 
     def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False,
-                 valu_products=False, full_powers=False):
+                 valu_products=False, full_powers=False, charpoly=False):
         import torch
 
         self.torch = torch
@@ -179,7 +182,7 @@ class DeviceBatch:
         if self.nreq == 0:
             return
         options = ((1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
-                   | (16 if full_powers else 0))
+                   | (16 if full_powers else 0) | (32 if charpoly else 0))
         reqs, pool = pack_jobs(jobs, options=options)
         with torch.cuda.device(self.device):
             self.pool = torch.from_numpy(pool.view(np.float64).copy()).to(self.device)

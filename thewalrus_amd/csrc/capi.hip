@@ -166,7 +166,7 @@ int describe(const wh_request& r, wh_job_dev& j) {
     j.E = r.E;
     j.N = (int)N;
     j.flags = (loop ? WH_F_LOOP : 0) | (odd ? WH_F_ODD : 0) | (glynn ? WH_F_GLYNN : 0) |
-              (r.real ? WH_F_REAL : 0);
+              (r.real ? WH_F_REAL : 0) | ((r.options & WH_OPT_CHARPOLY) ? WH_F_CHARPOLY : 0);
     return WH_OK;
 }
 
@@ -284,11 +284,13 @@ double term_flops(const wh_job_dev& j, const GroupKey& g) {
     // algorithmic FP64 flops of the products per term, as the kernels run them:
     // PM1: h - 1 products of n x n (8 flops per complex multiply-add, 2 per real one), or
     // ceil(h/2) - 1 with the half-products formulation;
-    // GENERIC: (min(h + 1, m) - 2) products of m x m with m = 2 * nnz(delta) <= 2E (upper bound);
+    // GENERIC: h - 1 products of m x m with m = 2 * nnz(delta) <= 2E (upper bound), or
+    // min(h + 1, m) - 2 with the characteristic-polynomial extension;
     // PM1S: the symmetric powers need the m (m + 1) / 2 entries of the upper triangle only
     int h = j.N / 2;
     int m = 2 * j.E;
     int np = std::min(h + 1, m) - 2;
+    if (g.kind == KIND_GENERIC && !(j.flags & WH_F_CHARPOLY)) np = h - 1;  // every power formed
     if (all_ones_kind(g.kind) && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
     double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;

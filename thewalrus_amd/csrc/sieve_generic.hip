@@ -27,6 +27,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
     const bool loop = (J.flags & WH_F_LOOP) != 0;
     const bool odd = (J.flags & WH_F_ODD) != 0;
     const bool glynn = (J.flags & WH_F_GLYNN) != 0;
+    const bool charpoly = (J.flags & WH_F_CHARPOLY) != 0;
     const int h = N / 2;
     const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
     int64_t t1 = t0 + J.chunk_terms;
@@ -125,12 +126,16 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
             }
         }
 
-        // charpoly.py:302-327 powertrace(H, h + 1)
+        // charpoly.py:302-327 powertrace(H, h + 1).  The reference forms powers up to
+        // B^(min(h + 1, m) - 1) and, when h + 1 > m, the remaining traces from the characteristic
+        // polynomial (a serial Householder reduction and La Budde per term); by default this
+        // kernel forms every power up to B^h directly instead (the same traces).
         const int nt = h + 1;
+        const bool extend = charpoly && nt > m;
         if (m == 0) {
             for (int kk = lane; kk <= h; kk += WH_WAVE) L.pt[kk] = cmk(0.0, 0.0);
         } else {
-            int min_val = nt < m ? nt : m;
+            int min_val = extend ? m : nt;
             cplx tr = trace_lds(L.H, m, MM, lane);
             if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
             const cplx* cur = L.H;
@@ -144,7 +149,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
                 cur = dst;
             }
             __syncthreads();
-            if (nt > m) {
+            if (extend) {
                 hessenberg_lds(L, m, MM, lane);
                 labudde_lds(L, m, MM, lane);
                 if (lane == 0) {

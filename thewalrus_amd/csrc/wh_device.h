@@ -31,6 +31,7 @@ struct __attribute__((aligned(16))) wh_job_dev {
 #define WH_F_GLYNN 4
 #define WH_F_REAL 8
 #define WH_F_MTL 16   // montrealer / loop montrealer terms (PM1 kernel only)
+#define WH_F_CHARPOLY 32  // general kernel: traces above p_(m-1) by the characteristic polynomial
 
 // One loop_hafnian_batch request (thewalrus/loop_hafnian_batch.py:52, :128; with nD > 1
 // loop_hafnian_batch_gamma.py:53, :132): the reordered matrix and nD diagonals, edge
