@@ -152,43 +152,24 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
         }
         __syncthreads();
 
-        // charpoly.py:302-327 powertrace(AX_S, he + 1); the Hessenberg reduction runs on a copy
-        // in Q so that AX_S stays intact for every diagonal's loop chains
+        // charpoly.py:302-327 powertrace(AX_S, he + 1).  When he + 1 > m the reference extends
+        // the traces by the characteristic polynomial (a serial Householder reduction and La
+        // Budde per term); this kernel forms every power up to AX_S^he directly (the same
+        // traces), as the general kernel does by default
         const int nt = he + 1;
         if (m == 0) {
             for (int kk = lane; kk <= he; kk += WH_WAVE) L.pt[kk] = cmk(0.0, 0.0);
         } else {
-            int min_val = nt < m ? nt : m;
             cplx tr = trace_lds(L.H, m, MM, lane);
             if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
             const cplx* cur = L.H;
-            for (int it = 0; it < min_val - 2; ++it) {
+            for (int it = 0; it < nt - 2; ++it) {
                 cplx* dst = (cur == L.P) ? L.Q : L.P;
                 matmul_lds(cur, L.H, dst, m, MM, lane);
                 __syncthreads();
                 cplx tk = trace_lds(dst, m, MM, lane);
                 if (lane == 0) L.pt[it + 2] = tk;
                 cur = dst;
-            }
-            __syncthreads();
-            if (nt > m) {
-                for (int e = lane; e < m * m; e += WH_WAVE) {
-                    int r = e / m, c = e - r * m;
-                    L.Q[r * MM + c] = L.H[r * MM + c];
-                }
-                __syncthreads();
-                GenLds Lc = L;
-                Lc.H = L.Q;
-                hessenberg_lds(Lc, m, MM, lane);
-                labudde_lds(Lc, m, MM, lane);
-                if (lane == 0) {
-                    for (int tt = min_val; tt < nt; ++tt) {
-                        cplx sacc = cmk(0.0, 0.0);
-                        for (int q = 0; q < m; ++q)
-                            sacc = csub(sacc, cmul(L.P[(m - 1) * MM + q], L.pt[tt - 1 - q]));
-                        L.pt[tt] = sacc;
-                    }
-                }
             }
         }
         __syncthreads();
