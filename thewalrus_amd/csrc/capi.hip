@@ -825,6 +825,9 @@ static int lhb_call(const double* A, const double* D, int64_t nD, int64_t twoE, 
     j.steps = steps;
     j.chunk_terms = chunk;
     j.nchunks = (int32_t)((steps + chunk - 1) / chunk);
+    // few chunks (small term counts): one workgroup per chunk and diagonal, so the fanout rows
+    // run in parallel (each repeats the chunk's shared per-term traces)
+    j.dsplit = (nD > 1 && j.nchunks < 1024) ? (int32_t)nD : 1;
 
     int dev;
     int rc = current_device(&dev);
@@ -850,8 +853,8 @@ static int lhb_call(const double* A, const double* D, int64_t nD, int64_t twoE, 
     memcpy(c->h_stage + job_bytes, A, nA * 16);
     memcpy(c->h_stage + job_bytes + nA * 16, D, nDv * 16);
     HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, in_bytes, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(wh_launch_lhb((const wh_lhb_job*)c->d_stage, 1, c->d_stage + job_bytes, c->d_partials, j.nchunks,
-                          (int)twoE, c->d_stage + out_at, c->stream));
+    HIP_TRY(wh_launch_lhb((const wh_lhb_job*)c->d_stage, 1, c->d_stage + job_bytes, c->d_partials,
+                          (int64_t)j.nchunks * j.dsplit, (int)twoE, c->d_stage + out_at, c->stream));
     HIP_TRY(hipMemcpyAsync(c->h_stage + out_at, c->d_stage + out_at, nout * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     memcpy(out, c->h_stage + out_at, nout * 16);

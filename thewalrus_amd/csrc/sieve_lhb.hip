@@ -74,7 +74,8 @@ __device__ cplx* coeff_series(cplx* oldc, cplx* newc, int top, bool odd_form, co
 }  // namespace
 
 // partials[J.part_off + chunk * nD * L + kd * L + q] = (sum_re, err_re, sum_im, err_im) of
-// output q of diagonal kd.
+// output q of diagonal kd.  Block chunk_base + chunk * dsplit + g takes the chunk's terms for
+// diagonal group g (dsplit groups of ceil(nD / dsplit) diagonals).
 extern "C" __global__ void __launch_bounds__(WH_WAVE)
 wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
              double4* __restrict__ partials, int64_t block_offset, int MM) {
@@ -92,7 +93,13 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
     const bool oddv = (J.flags & WH_LHB_ODD) != 0;
     const bool glynn = (J.flags & WH_LHB_GLYNN) != 0;
     const int he = nmax / 2;  // f_loop(n_max) order; powertrace(AX, n_max // 2 + 1)
-    const int64_t chunk = b - J.chunk_base;
+    // block -> (chunk, diagonal group): the group's diagonals [kd_lo, kd_hi) share the chunk's
+    // terms with the chunk's other groups
+    const int64_t chunk = (b - J.chunk_base) / J.dsplit;
+    const int grp = (int)((b - J.chunk_base) % J.dsplit);
+    const int ndg = (J.nD + J.dsplit - 1) / J.dsplit;
+    const int kd_lo = grp * ndg;
+    const int kd_hi = min(J.nD, kd_lo + ndg);
     const int64_t t0 = chunk * J.chunk_terms;
     int64_t t1 = t0 + J.chunk_terms;
     if (t1 > J.steps) t1 = J.steps;
@@ -175,7 +182,7 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
         __syncthreads();
 
         const int start = 2 * k0 + (oddv ? 1 : 0);
-        for (int kd = 0; kd < nD; ++kd) {
+        for (int kd = kd_lo; kd < kd_hi; ++kd) {
             // get_Dsubmatrices (_hafnian.py:383-399) for diagonal kd
             const cplx* Dk = Dg + (int64_t)kd * twoE;
             if (lane < m) {
@@ -268,9 +275,11 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
         }
     }
     const int nout = nD * nL;
+    const int o_lo = kd_lo * nL, o_hi = kd_hi * nL;  // this group's outputs
     double4* out = partials + J.part_off + chunk * nout;
-    if (lane < nout) out[lane] = make_double4(acc0.sr, acc0.cr, acc0.si, acc0.ci);
-    if (lane + WH_WAVE < nout) out[lane + WH_WAVE] = make_double4(acc1.sr, acc1.cr, acc1.si, acc1.ci);
+    if (lane >= o_lo && lane < o_hi) out[lane] = make_double4(acc0.sr, acc0.cr, acc0.si, acc0.ci);
+    if (lane + WH_WAVE >= o_lo && lane + WH_WAVE < o_hi)
+        out[lane + WH_WAVE] = make_double4(acc1.sr, acc1.cr, acc1.si, acc1.ci);
 }
 
 // Folds each output's chunk partials in chunk order and applies 0.5^((n_fixed + q) // 2) for

@@ -51,7 +51,7 @@ struct __attribute__((aligned(16))) wh_lhb_job {
     int32_t flags;        // WH_LHB_* bits
     int32_t nchunks;
     int32_t nD;           // diagonals sharing Ax (loop_hafnian_batch_gamma: one per fanout row)
-    int32_t pad_;
+    int32_t dsplit;       // workgroups per chunk, each taking a contiguous group of the diagonals
     int32_t e[WH_EMAX];
 };
 
