@@ -96,6 +96,17 @@ __device__ void matmul_inplace_lds(const cplx* cur, const cplx* H, cplx* P, int 
     }
 }
 
+// x^T P y over the m x m matrix P (row stride MM), every lane holding a few entries; the whole
+// wave receives the sum.
+__device__ cplx bilinear_lds(const cplx* x, const cplx* P, const cplx* y, int m, int MM, int lane) {
+    cplx s = cmk(0.0, 0.0);
+    for (int e = lane; e < m * m; e += WH_WAVE) {
+        const int i = e / m, j = e - i * m;
+        cmac(s, cmul(x[i], P[i * MM + j]), y[j]);
+    }
+    return wave_csum(s);
+}
+
 __device__ cplx trace_lds(const cplx* X, int m, int MM, int lane) {
     cplx z = cmk(0.0, 0.0);
     if (lane < m) z = X[lane * MM + lane];

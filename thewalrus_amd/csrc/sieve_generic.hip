@@ -92,8 +92,15 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
         }
         __syncthreads();
 
-        // loop scalars, taken before powertrace may overwrite H (f_loop uses the unmutated AX_S)
-        if (loop && !odd) {
+        // traces above B^(m-1) by the characteristic polynomial (the reference's route, option)
+        // or by direct powers (default, powertrace below)
+        const int nt = h + 1;
+        const bool extend = charpoly && nt > m;
+        // even-N loop scalars XD AX^(i-1) D: with direct powers they are bilinear sums over the
+        // powers powertrace forms anyway (below); on the reference's route the row chain of
+        // f_loop, taken before the Householder reduction overwrites H
+        const bool chain_lv = loop && !odd && extend;
+        if (chain_lv) {
             // _hafnian.py:232-234: a_i gets XD A^(i-1) D / 2; XD <- XD @ AX_S (row chain)
             if (lane < m) L.v[lane] = L.XD[lane];
             __syncthreads();
@@ -130,11 +137,21 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
         // B^(min(h + 1, m) - 1) and, when h + 1 > m, the remaining traces from the characteristic
         // polynomial (a serial Householder reduction and La Budde per term); by default this
         // kernel forms every power up to B^h directly instead (the same traces).
-        const int nt = h + 1;
-        const bool extend = charpoly && nt > m;
+        const bool power_lv = loop && !odd && !extend;  // XD B^k D from each power B^k
         if (m == 0) {
-            for (int kk = lane; kk <= h; kk += WH_WAVE) L.pt[kk] = cmk(0.0, 0.0);
+            for (int kk = lane; kk <= h; kk += WH_WAVE) {
+                L.pt[kk] = cmk(0.0, 0.0);
+                if (power_lv) L.lv[kk] = cmk(0.0, 0.0);
+            }
         } else {
+            if (power_lv) {  // lv_1 = XD . D, lv_2 = XD B D
+                cplx d1 = wave_csum(lane < m ? cmul(L.XD[lane], L.D[lane]) : cmk(0.0, 0.0));
+                cplx d2 = bilinear_lds(L.XD, L.H, L.D, m, MM, lane);
+                if (lane == 0) {
+                    L.lv[1] = d1;
+                    if (h >= 2) L.lv[2] = d2;
+                }
+            }
             int min_val = extend ? m : nt;
             cplx tr = trace_lds(L.H, m, MM, lane);
             if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
@@ -146,6 +163,10 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
                 __syncthreads();
                 cplx tk = trace_lds(dst, m, MM, lane);
                 if (lane == 0) L.pt[it + 2] = tk;
+                if (power_lv && it + 3 <= h) {  // dst = B^(it + 2): lv_(it + 3)
+                    cplx dk = bilinear_lds(L.XD, dst, L.D, m, MM, lane);
+                    if (lane == 0) L.lv[it + 3] = dk;
+                }
                 cur = dst;
             }
             __syncthreads();
