@@ -96,11 +96,12 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
         // or by direct powers (default, powertrace below)
         const int nt = h + 1;
         const bool extend = charpoly && nt > m;
-        // even-N loop scalars XD AX^(i-1) D: with direct powers they are bilinear sums over the
-        // powers powertrace forms anyway (below); on the reference's route the row chain of
-        // f_loop, taken before the Householder reduction overwrites H
-        const bool chain_lv = loop && !odd && extend;
-        if (chain_lv) {
+        // loop scalars XD AX^(k-1) D (and oddVX AX^(k-1) D for odd N): with direct powers they
+        // are bilinear sums over the powers powertrace forms anyway (below); on the reference's
+        // route the chains of f_loop / f_loop_odd, taken before the Householder reduction
+        // overwrites H
+        const bool chain_lv = loop && extend;
+        if (chain_lv && !odd) {
             // _hafnian.py:232-234: a_i gets XD A^(i-1) D / 2; XD <- XD @ AX_S (row chain)
             if (lane < m) L.v[lane] = L.XD[lane];
             __syncthreads();
@@ -115,7 +116,7 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
                 if (lane < m) L.v[lane] = nv;
                 __syncthreads();
             }
-        } else if (odd) {
+        } else if (chain_lv && odd) {
             // _hafnian.py:268-275: even orders use XD . z, odd orders oddVX . z, z <- AX_S z
             if (lane < m) L.v[lane] = L.D[lane];
             __syncthreads();
@@ -137,19 +138,28 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
         // B^(min(h + 1, m) - 1) and, when h + 1 > m, the remaining traces from the characteristic
         // polynomial (a serial Householder reduction and La Budde per term); by default this
         // kernel forms every power up to B^h directly instead (the same traces).
-        const bool power_lv = loop && !odd && !extend;  // XD B^k D from each power B^k
+        const bool power_lv = loop && !extend;  // XD B^k D (and oddVX B^k D) from each power B^k
         if (m == 0) {
             for (int kk = lane; kk <= h; kk += WH_WAVE) {
                 L.pt[kk] = cmk(0.0, 0.0);
-                if (power_lv) L.lv[kk] = cmk(0.0, 0.0);
+                if (power_lv) L.lv[kk] = L.ov[kk] = cmk(0.0, 0.0);
             }
         } else {
-            if (power_lv) {  // lv_1 = XD . D, lv_2 = XD B D
+            if (power_lv) {  // orders 1 and 2: x . D and x B D for x = XD (and oddVX)
                 cplx d1 = wave_csum(lane < m ? cmul(L.XD[lane], L.D[lane]) : cmk(0.0, 0.0));
                 cplx d2 = bilinear_lds(L.XD, L.H, L.D, m, MM, lane);
+                cplx o1 = cmk(0.0, 0.0), o2 = cmk(0.0, 0.0);
+                if (odd) {
+                    o1 = wave_csum(lane < m ? cmul(L.OV[lane], L.D[lane]) : cmk(0.0, 0.0));
+                    o2 = bilinear_lds(L.OV, L.H, L.D, m, MM, lane);
+                }
                 if (lane == 0) {
                     L.lv[1] = d1;
-                    if (h >= 2) L.lv[2] = d2;
+                    L.ov[1] = o1;
+                    if (h >= 2) {
+                        L.lv[2] = d2;
+                        L.ov[2] = o2;
+                    }
                 }
             }
             int min_val = extend ? m : nt;
@@ -163,9 +173,13 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
                 __syncthreads();
                 cplx tk = trace_lds(dst, m, MM, lane);
                 if (lane == 0) L.pt[it + 2] = tk;
-                if (power_lv && it + 3 <= h) {  // dst = B^(it + 2): lv_(it + 3)
+                if (power_lv && it + 3 <= h) {  // dst = B^(it + 2): order it + 3
                     cplx dk = bilinear_lds(L.XD, dst, L.D, m, MM, lane);
-                    if (lane == 0) L.lv[it + 3] = dk;
+                    cplx ok = odd ? bilinear_lds(L.OV, dst, L.D, m, MM, lane) : cmk(0.0, 0.0);
+                    if (lane == 0) {
+                        L.lv[it + 3] = dk;
+                        L.ov[it + 3] = ok;
+                    }
                 }
                 cur = dst;
             }
