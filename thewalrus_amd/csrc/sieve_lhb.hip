@@ -159,16 +159,59 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
         }
         __syncthreads();
 
+        // a group of one diagonal takes its loop scalars from the powers below: XD B^(k-1) D
+        // (both f_loop's row chain and f_loop_odd's XD . z), oddVX B^(k-1) D and oddVX0 B^(k-1) D
+        const bool one_diag = kd_hi - kd_lo == 1;
+        if (one_diag) {
+            const cplx* Dk = Dg + (int64_t)kd_lo * twoE;  // get_Dsubmatrices (_hafnian.py:383-399)
+            if (lane < m) {
+                int rr = lane < s ? L.S[lane] : L.S[lane - s] + E;
+                L.D[lane] = Dk[rr];
+                int c = lane < s ? lane : lane - s;
+                int src = lane < s ? L.S[c] + E : L.S[c];
+                L.XD[lane] = cmulr(Dk[src], (double)L.dS[c]);
+            }
+            __syncthreads();
+        }
+        // order k of the loop scalars from B^(k-1) (P = B^(k-1), k <= he)
+        auto scalars_from = [&](const cplx* P, int k) {
+            cplx a = bilinear_lds(L.XD, P, L.D, m, MM, lane);
+            cplx o = bilinear_lds(L.OV, P, L.D, m, MM, lane);
+            cplx o0 = odd0 ? bilinear_lds(X.OV0, P, L.D, m, MM, lane) : cmk(0.0, 0.0);
+            if (lane == 0) {
+                X.lr[k] = a;
+                L.lv[k] = a;
+                L.ov[k] = o;
+                X.ov0[k] = o0;
+            }
+        };
+
         // charpoly.py:302-327 powertrace(AX_S, he + 1).  When he + 1 > m the reference extends
         // the traces by the characteristic polynomial (a serial Householder reduction and La
         // Budde per term); this kernel forms every power up to AX_S^he directly (the same
         // traces), as the general kernel does by default
         const int nt = he + 1;
         if (m == 0) {
-            for (int kk = lane; kk <= he; kk += WH_WAVE) L.pt[kk] = cmk(0.0, 0.0);
+            for (int kk = lane; kk <= he; kk += WH_WAVE) {
+                L.pt[kk] = cmk(0.0, 0.0);
+                if (one_diag) X.lr[kk] = L.lv[kk] = L.ov[kk] = X.ov0[kk] = cmk(0.0, 0.0);
+            }
         } else {
             cplx tr = trace_lds(L.H, m, MM, lane);
             if (lane == 0) { L.pt[0] = cmk((double)m, 0.0); L.pt[1] = tr; }
+            if (one_diag) {  // order 1: x . D; order 2: x B D
+                cplx a = wave_csum(lane < m ? cmul(L.XD[lane], L.D[lane]) : cmk(0.0, 0.0));
+                cplx o = wave_csum(lane < m ? cmul(L.OV[lane], L.D[lane]) : cmk(0.0, 0.0));
+                cplx o0 = cmk(0.0, 0.0);
+                if (odd0) o0 = wave_csum(lane < m ? cmul(X.OV0[lane], L.D[lane]) : cmk(0.0, 0.0));
+                if (lane == 0) {
+                    X.lr[1] = a;
+                    L.lv[1] = a;
+                    L.ov[1] = o;
+                    X.ov0[1] = o0;
+                }
+                if (he >= 2) scalars_from(L.H, 2);
+            }
             const cplx* cur = L.H;
             for (int it = 0; it < nt - 2; ++it) {
                 cplx* dst = (cur == L.P) ? L.Q : L.P;
@@ -176,6 +219,7 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
                 __syncthreads();
                 cplx tk = trace_lds(dst, m, MM, lane);
                 if (lane == 0) L.pt[it + 2] = tk;
+                if (one_diag && it + 3 <= he) scalars_from(dst, it + 3);  // dst = B^(it + 2)
                 cur = dst;
             }
         }
@@ -185,6 +229,7 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
         for (int kd = kd_lo; kd < kd_hi; ++kd) {
             // get_Dsubmatrices (_hafnian.py:383-399) for diagonal kd
             const cplx* Dk = Dg + (int64_t)kd * twoE;
+            if (!one_diag) {
             if (lane < m) {
                 int rr = lane < s ? L.S[lane] : L.S[lane - s] + E;
                 L.D[lane] = Dk[rr];
@@ -229,6 +274,7 @@ wh_sieve_lhb(const wh_lhb_job* __restrict__ jobs, int njobs, const cplx* __restr
                 if (lane < m) L.v[lane] = nv;
                 __syncthreads();
             }
+            }  // !one_diag: a single diagonal's scalars came from the powers
 
             // coefficient series; f_loop(n_max) is kept in X.fe while the others reuse the rows
             cplx* c0 = L.comb;
