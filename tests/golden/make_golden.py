@@ -2,6 +2,8 @@
 
     python tests/golden/make_golden.py small   # seconds: configs 1, 2, 4, 5 samples
     python tests/golden/make_golden.py n40     # minutes: config 3 and structured n = 40
+    python tests/golden/make_golden.py band    # ~40 min: random n = 30..38, structured 32..44
+    python tests/golden/make_golden.py n48     # ~2 h: structured n = 48 (FP64 reference values)
 
 Truth values come from the oracle's double-double restatement (precision="dd"); for n = 40
 the per-term characteristic-polynomial route (variant=1) keeps the cost to minutes.  The
@@ -28,24 +30,34 @@ def _c(z):
     return np.complex128(z)
 
 
-def structured(n_half, loop, seed):
+def structured(n_half, loop, seed, real=False):
     """A = W P (A1 (+) A2) P^T W: lhaf(A) = lhaf(A1) lhaf(A2) prod(w).  Off-diagonal entries
-    scale by w_i w_j, loop weights by w_i (each vertex is covered once)."""
+    scale by w_i w_j, loop weights by w_i (each vertex is covered once).  ``real`` draws real
+    blocks and real w (signs and magnitudes) and returns a float64 matrix."""
     rng = np.random.default_rng(seed)
     s = np.sqrt(1.0 / (4 * n_half))
     blocks = []
     for _ in range(2):
-        B = rng.normal(scale=s, size=(n_half, n_half)) + 1j * rng.normal(scale=s, size=(n_half, n_half))
+        if real:
+            B = rng.normal(scale=2 * s, size=(n_half, n_half))
+        else:
+            B = rng.normal(scale=s, size=(n_half, n_half)) + 1j * rng.normal(scale=s, size=(n_half, n_half))
         X = (B + B.T) / 2
         if loop:
-            X[np.diag_indices(n_half)] = (rng.normal(size=n_half) + 1j * rng.normal(size=n_half)) / np.sqrt(2)
+            if real:
+                X[np.diag_indices(n_half)] = rng.normal(size=n_half)
+            else:
+                X[np.diag_indices(n_half)] = (rng.normal(size=n_half) + 1j * rng.normal(size=n_half)) / np.sqrt(2)
         blocks.append(X)
     n = 2 * n_half
-    A = np.zeros((n, n), dtype=np.complex128)
+    A = np.zeros((n, n), dtype=np.float64 if real else np.complex128)
     A[:n_half, :n_half], A[n_half:, n_half:] = blocks
     perm = rng.permutation(n)
     A = A[np.ix_(perm, perm)]
-    w = np.exp(1j * rng.uniform(0, 2 * np.pi, size=n)) * rng.uniform(0.8, 1.25, size=n)
+    if real:
+        w = rng.choice([-1.0, 1.0], size=n) * rng.uniform(0.8, 1.25, size=n)
+    else:
+        w = np.exp(1j * rng.uniform(0, 2 * np.pi, size=n)) * rng.uniform(0.8, 1.25, size=n)
     d = np.diag(A).copy()
     A = (w[:, None] * A) * w[None, :]
     A[np.diag_indices(n)] = w * d
@@ -104,9 +116,65 @@ def n40():
     np.savez_compressed(os.path.join(HERE, "golden_n40.npz"), **out)
 
 
+def _random(n, loop, seed, real=False):
+    """Configuration-5-shaped input: A = (B + B^T)/2, B ~ CN(0, 1/n) (real: N(0, 1/n))."""
+    rng = np.random.default_rng(seed)
+    s = np.sqrt(1.0 / (2 * n))
+    if real:
+        B = rng.normal(scale=2 * s, size=(n, n))
+    else:
+        B = rng.normal(scale=s, size=(n, n)) + 1j * rng.normal(scale=s, size=(n, n))
+    return (B + B.T) / 2
+
+
+def band():
+    """The tile classes of sides 26..48 (8T >= n, T = 4, 5, 6): random n = 30..38 (complex, and
+    real n = 30, 34) with double-double truth from the per-term characteristic-polynomial route,
+    and structured direct sums n = 32, 36, 44 (complex) and 42 (real) with exact truth.  Every
+    case also stores the FP64 value of the reference algorithm (variant 0)."""
+    out = {}
+    t0 = time.time()
+    cases = [(n, lp, False) for n in (30, 32, 34, 36, 38) for lp in (False, True)]
+    cases += [(n, lp, True) for n in (30, 34) for lp in (False, True)]
+    for i, (n, lp, real) in enumerate(cases):
+        A = _random(n, lp, 3000 + i, real)
+        key = "r%d_%s_%s" % (n, "loop" if lp else "plain", "real" if real else "cplx")
+        out[key + "_A"] = A
+        out[key + "_truth"] = _c(O.hafnian_sieve(A, loop=lp, precision="dd", variant=1))
+        out[key + "_ref64"] = _c(O.hafnian_sieve(A, loop=lp, precision="double"))
+        print(key, "%.0fs" % (time.time() - t0), flush=True)
+    for n, real in ((32, False), (36, False), (44, False), (42, True)):
+        for lp in (False, True):
+            S, t = structured(n // 2, lp, 4000 + 2 * n + lp, real)
+            key = "s%d_%s_%s" % (n, "loop" if lp else "plain", "real" if real else "cplx")
+            out[key + "_A"] = S
+            out[key + "_truth"] = t
+            out[key + "_ref64"] = _c(O.hafnian_sieve(S, loop=lp, precision="double"))
+            print(key, "%.0fs" % (time.time() - t0), flush=True)
+    np.savez_compressed(os.path.join(HERE, "golden_band.npz"), **out)
+
+
+def n48():
+    """Structured n = 48 (plain and loop, complex): exact truth, FP64 reference values."""
+    out = {}
+    t0 = time.time()
+    for lp in (False, True):
+        S, t = structured(24, lp, 4096 + lp)
+        key = "s48_%s_cplx" % ("loop" if lp else "plain")
+        out[key + "_A"] = S
+        out[key + "_truth"] = t
+        out[key + "_ref64"] = _c(O.hafnian_sieve(S, loop=lp, precision="double"))
+        print(key, "%.0fs" % (time.time() - t0), flush=True)
+        np.savez_compressed(os.path.join(HERE, "golden_n48.npz"), **out)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "small"
     if what in ("small", "all"):
         small()
     if what in ("n40", "all"):
         n40()
+    if what in ("band", "all"):
+        band()
+    if what in ("n48", "all"):
+        n48()

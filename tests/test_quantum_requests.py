@@ -65,6 +65,31 @@ def test_conversions_roundtrip():
     assert not q.is_valid_cov(np.diag([0.5, 0.5]))  # violates the uncertainty relation
 
 
+@pytest.mark.parametrize("hbar", [2.0, 1.0])
+def test_conversions_closed_forms(hbar):
+    """Q of textbook states from <a^dagger a> and <a a>: thermal with mean photon number nb
+    gives Q = (nb + 1) I; x-squeezed vacuum (Var x = hbar/2 e^(-2r)) has <a^dagger a> =
+    sinh(r)^2 and <a a> = -sinh(r) cosh(r); a coherent state's alpha is (x + i p)/sqrt(2 hbar)."""
+    nb = 0.7
+    assert np.allclose(q.Qmat(hbar / 2 * (2 * nb + 1) * np.identity(4), hbar=hbar), (nb + 1) * np.identity(4))
+    r = 0.4
+    V = hbar / 2 * np.diag([np.exp(-2 * r), np.exp(2 * r)])
+    Q = q.Qmat(V, hbar=hbar)
+    n_, m_ = np.sinh(r) ** 2, -np.sinh(r) * np.cosh(r)
+    assert np.allclose(Q, [[n_ + 1, np.conj(m_)], [m_, n_ + 1]])
+    assert np.allclose(q.Covmat(Q, hbar=hbar), V)
+    # a pure state's A is B (+) B^* with B the amplitude matrix, here B = -tanh(r)
+    A = q.Amat(V, hbar=hbar)
+    assert np.allclose(A, [[-np.tanh(r), 0], [0, -np.tanh(r)]])
+    x, p = 0.3, -1.1
+    beta = q.complex_to_real_displacements(np.array([x, p]), hbar=hbar)
+    assert np.allclose(beta, [(x + 1j * p) / np.sqrt(2 * hbar), (x - 1j * p) / np.sqrt(2 * hbar)])
+    assert np.allclose(q.real_to_complex_displacements(beta, hbar=hbar), [x, p])
+    # vacuum prefactor of a coherent state: |<0|alpha>|^2 = exp(-|alpha|^2)
+    assert np.isclose(q._prefactor(np.array([x, p]), hbar / 2 * np.identity(2), hbar=hbar),
+                      np.exp(-abs(beta[0]) ** 2))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("t", DISP)
 def test_density_matrix_element_disp(t):

@@ -37,91 +37,112 @@ __all__ = [
 ]
 
 
+def _swap_halves(N):
+    """The 2N x 2N permutation that exchanges the first and second N coordinates."""
+    P = np.zeros((2 * N, 2 * N))
+    idx = np.arange(2 * N)
+    P[idx, (idx + N) % (2 * N)] = 1.0
+    return P
+
+
+def _ladder_basis(N):
+    """W with (a, a^dagger) = W (x, p) / sqrt(hbar) for a = (x + i p) / sqrt(2 hbar): the unitary
+    change from quadratures to ladder operators, W = [[I, iI], [I, -iI]] / sqrt(2)."""
+    W = np.zeros((2 * N, 2 * N), dtype=np.complex128)
+    r = np.arange(N)
+    W[r, r] = W[r + N, r] = 1.0
+    W[r, r + N] = 1j
+    W[r + N, r + N] = -1j
+    return W / np.sqrt(2.0)
+
+
 def Xmat(N):
     """[[0, I_N], [I_N, 0]] (conversions.py:55-67)."""
-    I = np.identity(N)
-    O = np.zeros_like(I)
-    return np.block([[O, I], [I, O]])
+    return _swap_halves(N)
 
 
 def sympmat(N, dtype=np.float64):
-    """The symplectic form [[0, I_N], [-I_N, 0]]."""
-    I = np.identity(N, dtype=dtype)
-    O = np.zeros_like(I, dtype=dtype)
-    return np.block([[O, I], [-I, O]])
+    """The symplectic form Omega = [[0, I_N], [-I_N, 0]]."""
+    Om = np.zeros((2 * N, 2 * N), dtype=dtype)
+    r = np.arange(N)
+    Om[r, r + N] = 1
+    Om[r + N, r] = -1
+    return Om
 
 
 def Qmat(cov, hbar=2):
-    """Husimi Q matrix of a Gaussian state from its xp-covariance (conversions.py:70-96)."""
-    N = len(cov) // 2
-    I = np.identity(N)
-    x = cov[:N, :N] * 2 / hbar
-    xp = cov[:N, N:] * 2 / hbar
-    p = cov[N:, N:] * 2 / hbar
-    aidaj = (x + p + 1j * (xp - xp.T) - 2 * I) / 4   # <a_i^dagger a_j>
-    aiaj = (x - p + 1j * (xp + xp.T)) / 4            # <a_i a_j>
-    return np.block([[aidaj, aiaj.conj()], [aiaj, aidaj.conj()]]) + np.identity(2 * N)
+    """Husimi Q matrix of a Gaussian state from its xp-covariance (conversions.py:70-96).
+
+    In the basis W* (the conjugate of the ladder basis), Q = W* cov W^T / hbar + I/2: its
+    top-left block is <a_i^dagger a_j> + delta_ij, its bottom-left block <a_i a_j>, and the
+    vacuum (cov = hbar/2 I) gives Q = I."""
+    cov = np.asarray(cov)
+    Wc = _ladder_basis(len(cov) // 2).conj()
+    return Wc @ cov @ Wc.conj().T / hbar + 0.5 * np.identity(len(cov))
 
 
 def Covmat(Q, hbar=2):
-    """xp-covariance from the Husimi Q matrix; inverse of :func:`Qmat` (conversions.py:99-121)."""
-    n = len(Q) // 2
-    I = np.identity(n)
-    N = Q[0:n, 0:n] - I
-    M = Q[n:2 * n, 0:n]
-    mm11a = 2 * (N.real + M.real) + np.identity(n)
-    mm22a = 2 * (N.real - M.real) + np.identity(n)
-    mm12a = 2 * (M.imag + N.imag)
-    return (hbar / 2) * np.block([[mm11a, mm12a], [mm12a.T, mm22a]])
+    """xp-covariance from the Husimi Q matrix, the inverse map of :func:`Qmat`
+    (conversions.py:99-121): cov = hbar W^T (Q - I/2) W*, real for a physical Q."""
+    Q = np.asarray(Q)
+    Wc = _ladder_basis(len(Q) // 2).conj()
+    return (hbar * (Wc.conj().T @ (Q - 0.5 * np.identity(len(Q))) @ Wc)).real
 
 
 def Amat(cov, hbar=2, cov_is_qmat=False):
-    """A = X (I - Q^-1)^*, whose hafnians give the photon-number statistics (conversions.py:124-150)."""
-    N = len(cov) // 2
-    Q = cov if cov_is_qmat else Qmat(cov, hbar=hbar)
-    return Xmat(N) @ (np.identity(2 * N) - np.linalg.inv(Q)).conj()
+    """A = X (I - Q^-1)^*, whose hafnians give the photon-number statistics (conversions.py:124-150);
+    X only exchanges the two row blocks."""
+    Q = np.asarray(cov) if cov_is_qmat else Qmat(cov, hbar=hbar)
+    N = len(Q) // 2
+    M = np.conj(np.identity(2 * N) - np.linalg.inv(Q))
+    return np.concatenate([M[N:], M[:N]], axis=0)
 
 
 def complex_to_real_displacements(mu, hbar=2):
-    """(alpha, alpha^*) from the quadrature means (conversions.py:153-169)."""
-    N = len(mu) // 2
-    alpha = (mu[:N] + 1j * mu[N:]) / np.sqrt(2 * hbar)
-    return np.concatenate([alpha, alpha.conj()])
+    """(alpha, alpha^*) from the quadrature means mu = (x, p), alpha = (x + i p) / sqrt(2 hbar)
+    (conversions.py:153-169)."""
+    mu = np.asarray(mu)
+    return _ladder_basis(len(mu) // 2) @ mu / np.sqrt(hbar)
 
 
 def real_to_complex_displacements(beta, hbar=2):
-    """Quadrature means from (alpha, alpha^*) (conversions.py:172-187)."""
+    """Quadrature means from (alpha, alpha^*), the inverse map of
+    :func:`complex_to_real_displacements` (conversions.py:172-187); only alpha is read."""
+    beta = np.asarray(beta)
     N = len(beta) // 2
-    alpha = beta[0:N]
-    return np.sqrt(2 * hbar) * np.concatenate([alpha.real, alpha.imag])
+    full = np.concatenate([beta[:N], np.conj(beta[:N])])
+    return np.sqrt(hbar) * (_ladder_basis(N).conj().T @ full).real
 
 
 def is_valid_cov(cov, hbar=2, rtol=1e-05, atol=1e-08):
-    """Square, symmetric, even, and cov + i hbar/2 Omega >= 0 (gaussian_checks.py:12-56)."""
-    (n, m) = cov.shape
-    if n != m or n % 2 != 0:
+    """Square, symmetric, even-sized, and obeying the uncertainty relation
+    cov + i hbar/2 Omega >= 0 up to ``atol`` (gaussian_checks.py:12-56)."""
+    rows, cols = cov.shape
+    if rows != cols or rows % 2:
         return False
-    if not np.allclose(cov, np.transpose(cov), rtol=rtol, atol=atol):
+    if not np.allclose(cov, cov.T, rtol=rtol, atol=atol):
         return False
-    vals = np.linalg.eigvalsh(cov + 0.5j * hbar * sympmat(n // 2))
-    vals[np.abs(vals) < atol] = 0.0
-    return bool(np.all(vals >= 0))
+    lam = np.linalg.eigvalsh(cov + 0.5j * hbar * sympmat(rows // 2))
+    return bool(np.all((lam > -atol) | (np.abs(lam) < atol)))
 
 
 def is_pure_cov(cov, hbar=2, rtol=1e-05, atol=1e-08):
-    """Valid and of unit purity (gaussian_checks.py:59-77)."""
-    if is_valid_cov(cov, hbar=hbar, rtol=rtol, atol=atol):
-        purity = 1 / np.sqrt(np.linalg.det(2 * cov / hbar))
-        if np.allclose(purity, 1.0, rtol=rtol, atol=atol):
-            return True
-    return False
+    """Valid and of unit purity (gaussian_checks.py:59-77).  The purity of a Gaussian state is
+    det(2 cov / hbar)^(-1/2)."""
+    if not is_valid_cov(cov, hbar=hbar, rtol=rtol, atol=atol):
+        return False
+    purity = np.linalg.det(2.0 * np.asarray(cov) / hbar) ** -0.5
+    return bool(np.allclose(purity, 1.0, rtol=rtol, atol=atol))
 
 
 def _prefactor(mu, cov, hbar=2):
-    """exp(-beta Q^-1 beta^* / 2) / sqrt(det Q) (fock_tensors.py:566-581)."""
+    """<0|D(-alpha) rho D(alpha)|0>-type normalisation of a Gaussian element
+    (fock_tensors.py:566-581): exp(-beta^T Q^-1 beta^* / 2) / sqrt(det Q) with beta = (alpha,
+    alpha^*); the quadratic form by a linear solve."""
     Q = Qmat(cov, hbar=hbar)
     beta = complex_to_real_displacements(mu, hbar=hbar)
-    return np.exp(-0.5 * beta @ np.linalg.inv(Q) @ beta.conj()) / np.sqrt(np.linalg.det(Q))
+    quad = beta @ np.linalg.solve(Q, np.conj(beta))
+    return np.exp(-0.5 * quad) / np.sqrt(np.linalg.det(Q))
 
 
 def _few_repetitions(rpt):
