@@ -1,26 +1,36 @@
-"""Benchmark: hafnians/s on the mixed n in [8, 40] complex128 request stream, one MI355X.
+"""Benchmark: hafnians/s and p50/p99 latency on the mixed n in [8, 40] complex128 request stream
+served on one MI355X, beside per-call GPU dispatch.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests R]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--step-requests R] [--load L]
 
-Workload (the metric's): the serving stream of the benchmark configurations -- even n
-uniform in [8, 40], half of them loop hafnians, entries complex N(0, 1/n) symmetrised
-(``thewalrus_amd.workloads.config5``, seed 141).  A step is one pass of the sieve over a batch
-of R requests of that stream with every request's reordered matrix already resident in HBM;
-``value`` = R * K / (wall time of K steps).  The same JSON line also carries:
-  * per_call: the same requests through the drop-in ``hafnian()`` one call at a time
-    (synchronous, host buffers), and the batched / per-call throughput ratio;
-  * serving: the continuous-batching server fed by Poisson arrivals at 0.5x and 0.9x of the
-    measured batched capacity: p50 / p99 latency (submit to result) and throughput;
-  * small_stream: batched vs per-call on the same stream restricted to n <= 24, where
-    per-call dispatch cost matters;
+Workload (the metric's): the serving stream of the benchmark configurations -- Poisson
+arrivals, even n uniform in [8, 40], half of them loop hafnians, entries complex N(0, 1/n)
+symmetrised (``thewalrus_amd.workloads.config5``, seed 141).  A step is R = 500 consecutive
+requests of that stream; K = 20 steps serve its 10^4 requests once.  Before the timed region the
+same K R requests run as K resident device batches of R (the batched capacity C).  The timed
+region then runs the K steps as one continuous stream: each request is submitted to the
+continuous-batching server at its Poisson arrival time (offered rate ``qps`` = L x C, L = 1 by
+default), host preparation (validation, pairing, reorder) and submission inside the region, and
+the region ends when the last result is back.  ``value`` = K R / (wall time of the region):
+served hafnians/s.  p50 / p99 are the latencies (submit to result) of those same K R requests.
+Warm-up: the W steps of a warm-up stream (seed 142) run batched and served first.
+
+The same JSON line also carries:
+  * batched: the batched capacity C (one plan per step, inputs already in HBM) and served / C;
+  * serving_half_load: latencies of the first two steps served at 0.5 C;
+  * warmup_step_served_closed_loop / warmup_step_batched: one warm-up step submitted to the
+    server at once, and the same step as one resident batch;
+  * per_call: the same K R requests through the drop-in ``hafnian()`` one call at a time
+    (synchronous, host buffers), served / per-call and batched / per-call;
+  * small_stream: batched vs per-call on the stream restricted to n <= 24, where per-call
+    dispatch cost matters;
   * configs: the other benchmark configurations (1: n = 20 per call; 2: 4096 real 24 x 24 in
     one launch; 3: n = 40 loop single solve; 4: 1024 hafnian_repeated requests);
   * next_rows: the GBS samplers' batched loop hafnians (loop_hafnian_batch, _gamma) and the
     montrealer at 16 and 20 modes, per call, with the CPU oracle beside them;
   * roofline: FP64 product flops of the dominant kernel / its HIP-event time (same stream);
-  * cpu_baseline: the CPU oracle (C++ restatement of the reference algorithm, OpenMP over
-    terms, all host cores) on a bounded sample of the stream, scaled to hafnians/s by
-    algorithmic work.
+  * cpu_baseline: the CPU restatement of the reference algorithm on host cores, timed per n on
+    a bounded sample of terms, summed over the stream's requests.
 """
 import argparse
 import json
@@ -59,21 +69,6 @@ def per_call(reqs, tw):
     return len(reqs) / (time.perf_counter() - t0), vals
 
 
-def serve_run(reqs, qps, Server):
-    arrivals = W.arrivals(len(reqs), qps)
-    with Server(slots=4, slot_pool_mb=16) as srv:
-        t0 = time.perf_counter()
-        for (A, l), ta in zip(reqs, arrivals):
-            wait = ta - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            srv.submit(A, loop=l)
-        srv.drain()
-        s = srv.stats.summary()
-    s["offered_qps"] = qps
-    return s
-
-
 def serve_closed(reqs, Server, reps=3):
     """Every request submitted at once to a server, after one untimed pass (first launch of
     each kernel class), ``reps`` times: (median hafnians/s from the first submission to the
@@ -90,45 +85,72 @@ def serve_closed(reqs, Server, reps=3):
     return float(np.median(rates)), vals
 
 
-def cpu_baseline(reqs, budget_s=12.0):
-    """Oracle on a sample of the stream (largest n <= 32 first), parallel over terms (OpenMP,
-    one request at a time) and over requests (one single-threaded sieve per thread); hafnians/s
-    of the whole stream from the faster measured algorithmic flop rate."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_class_seconds(A, loop, cores, target_s=0.25):
+    """Seconds per hafnian of one (n, loop) class on the timed CPU build (all host threads):
+    n >= 24, over terms on a bounded term range of one request scaled by its term count (every
+    term of an all-ones request costs the same); n < 24, over requests (one sieve per thread)."""
+    from oracle import oracle as O
 
+    n = A.shape[0]
+    h = n // 2
+    steps = 2 ** (h - 1)
+    Ax, Dx = O.ct_prepare(A, loop)
+    if n >= 24:
+        k = min(steps, 64 * cores)
+        O.ct_sieve(Ax, Dx, 0, k, cores)
+        t0 = time.perf_counter()
+        O.ct_sieve(Ax, Dx, 0, k, cores)
+        dt = time.perf_counter() - t0
+        k2 = int(min(steps, max(k, k * target_s / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        O.ct_sieve(Ax, Dx, 0, k2, cores)
+        return (time.perf_counter() - t0) / k2 * steps, "%d of %d terms" % (k2, steps)
+    count = 8 * cores
+    O.ct_sieve_many([Ax] * cores, None if Dx is None else [Dx] * cores, cores)
+    t0 = time.perf_counter()
+    O.ct_sieve_many([Ax] * count, None if Dx is None else [Dx] * count, cores)
+    dt = time.perf_counter() - t0
+    reps = int(max(1, target_s / max(dt, 1e-6)))
+    if reps > 1:
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            O.ct_sieve_many([Ax] * count, None if Dx is None else [Dx] * count, cores)
+        dt = (time.perf_counter() - t0) / reps
+    return dt / count, "%d requests x %d" % (count, reps)
+
+
+def cpu_baseline(reqs):
+    """The reference's CPU path, timed: the performance restatement (oracle/cpu_timing.cpp,
+    blocked complex products with FMA, OpenMP over terms as numba's prange, or over requests for
+    small n) per (n, loop) class of the stream on every host thread, each class on a bounded
+    sample; hafnians/s of the whole stream = requests / the sum of their class times."""
     from oracle import oracle as O
 
     cores = len(os.sched_getaffinity(0))
-    sample = sorted([r for r in reqs if r[0].shape[0] <= 32], key=lambda r: -r[0].shape[0])
-    done_w, spent, used = 0.0, 0.0, []
-    O.hafnian_sieve(np.ones((8, 8)), nthreads=cores)
-    for A, loop in sample:
-        if spent > budget_s:
-            break
-        t0 = time.perf_counter()
-        O.hafnian_sieve(A, loop=loop, nthreads=cores)
-        spent += time.perf_counter() - t0
-        done_w += W.w_glynn(A.shape[0])
-        used.append((A, loop))
-    rate_terms = done_w / spent
-    with ThreadPoolExecutor(cores) as ex:
-        t0 = time.perf_counter()
-        list(ex.map(lambda r: O.hafnian_sieve(r[0], loop=r[1], nthreads=1), used))
-        spent_mats = time.perf_counter() - t0
-    rate_mats = done_w / spent_mats
-    rate = max(rate_terms, rate_mats)
-    total_w = sum(W.w_glynn(A.shape[0]) for A, _ in reqs)
-    ns = [A.shape[0] for A, _ in used]
+    first = {}
+    for A, loop in reqs:
+        first.setdefault((A.shape[0], loop), A)
+    per = {}
+    notes = []
+    t_all = time.perf_counter()
+    for (n, loop) in sorted(first):
+        sec, how = cpu_class_seconds(first[(n, loop)], loop, cores)
+        per[(n, loop)] = sec
+        notes.append("n=%d%s %.4g ms (%s)" % (n, " loop" if loop else "", sec * 1e3, how))
+    spent = time.perf_counter() - t_all
+    total = sum(per[(A.shape[0], loop)] for A, loop in reqs)
+    _, isa = O.cpu_timing_lib()
     return {
-        "value": len(reqs) / (total_w / rate),
+        "value": len(reqs) / total,
         "unit": "hafnians/s",
         "cores": cores,
         "kind": "port",
-        "sample": "CPU oracle (C++ restatement of thewalrus _calc_hafnian/_calc_loop_hafnian) on %d stream "
-        "requests with n in [%d, %d]: OpenMP over terms, %d threads, %.1f s (%.3g flop/s of the reference "
-        "algorithm's products); over requests, %d threads, %.1f s (%.3g flop/s); the faster rate scaled to "
-        "the full stream by those flops"
-        % (len(used), min(ns), max(ns), cores, spent, rate_terms, cores, spent_mats, rate_mats),
+        "sample": "CPU restatement of the reference algorithm's CPU path (oracle/cpu_timing.cpp: per-term "
+        "products as register-blocked complex products with FMA, -O3 -march=%s, OpenMP over terms as numba's "
+        "prange; over requests for n < 24) on %d host threads; each (n, loop) class of the %d-request stream "
+        "timed on one request (a bounded range of its terms, scaled by the term count) in %.1f s in all; "
+        "stream time = sum over requests of their class time = %.0f s. Per class: %s"
+        % (isa, cores, len(reqs), spent, total, "; ".join(notes)),
     }
 
 
@@ -146,11 +168,12 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
         from oracle import oracle as O
 
         cores = len(os.sched_getaffinity(0))
-        O.hafnian_sieve(A1, nthreads=cores)
+        Ax1, _ = O.ct_prepare(A1, False)
+        O.ct_sieve(Ax1, None, 0, 512, cores)
         t0 = time.perf_counter()
         for _ in range(5):
-            O.hafnian_sieve(A1, nthreads=cores)
-        out["config1_n20_cpu_oracle_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+            O.ct_sieve(Ax1, None, 0, 512, cores)
+        out["config1_n20_cpu_ms"] = (time.perf_counter() - t0) / 5 * 1e3
 
     mats = W.config2()
     b2 = B.DeviceBatch([B.prepare(M) for M in mats])
@@ -164,27 +187,27 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
     }
     b2.close()
     if with_cpu:
-        # the same real n = 24 matrices on the CPU oracle (all host threads), 32 of them
+        # the same real n = 24 matrices on the timed CPU build (complex arithmetic, as the
+        # reference casts them), all host threads: over terms (one matrix at a time) and over
+        # matrices (one sieve per thread), the faster kept
         from oracle import oracle as O
 
-        from concurrent.futures import ThreadPoolExecutor
-
         cores = len(os.sched_getaffinity(0))
-        O.hafnian_sieve(mats[0], nthreads=cores)
+        prepped = [O.ct_prepare(M, False)[0] for M in mats[:8 * cores]]
+        O.ct_sieve(prepped[0], None, 0, 2048, cores)
         t0 = time.perf_counter()
-        for M in mats[:32]:  # OpenMP over terms, one matrix at a time
-            O.hafnian_sieve(M, nthreads=cores)
-        rate_terms = 32 / (time.perf_counter() - t0)
-        with ThreadPoolExecutor(cores) as ex:  # over matrices: one single-threaded sieve per thread
-            list(ex.map(lambda M: O.hafnian_sieve(M, nthreads=1), mats[:cores]))
-            t0 = time.perf_counter()
-            list(ex.map(lambda M: O.hafnian_sieve(M, nthreads=1), mats[:8 * cores]))
-            rate_mats = 8 * cores / (time.perf_counter() - t0)
+        for Ax in prepped[:16]:
+            O.ct_sieve(Ax, None, 0, 2048, cores)
+        rate_terms = 16 / (time.perf_counter() - t0)
+        O.ct_sieve_many(prepped[:cores], None, cores)
+        t0 = time.perf_counter()
+        O.ct_sieve_many(prepped, None, cores)
+        rate_mats = len(prepped) / (time.perf_counter() - t0)
         cpu_rate = max(rate_terms, rate_mats)
         row = out["config2_real24_batch"]
-        row["cpu_oracle_hafnians_per_s"] = cpu_rate
-        row["cpu_oracle_over_terms"] = rate_terms
-        row["cpu_oracle_over_matrices"] = rate_mats
+        row["cpu_hafnians_per_s"] = cpu_rate
+        row["cpu_over_terms"] = rate_terms
+        row["cpu_over_matrices"] = rate_mats
         row["cpu_cores"] = cores
         row["gpu_over_cpu"] = row["hafnians_per_s"] / cpu_rate
 
@@ -205,10 +228,18 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
     b3.close()
 
     reqs4 = W.config4()
-    b4 = B.DeviceBatch([B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in reqs4])
+    p4 = [B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in reqs4]
+    b4 = B.DeviceBatch(p4)
     b4.run(stream)
     ev, wall = timed_runs(b4, 3, torch, stream)
-    out["config4_repeated_batch"] = {"requests_per_s": 3 * len(reqs4) / wall, "kernel_ms": ev / 3 * 1e3}
+    wref = sum(w_rep(p.job) for p in p4 if p.on_device)
+    out["config4_repeated_batch"] = {
+        "requests_per_s": 3 * len(reqs4) / wall, "kernel_ms": ev / 3 * 1e3,
+        "kernel_tflops": b4.flops / (ev / 3) / 1e12, "frac_fp64_peak": b4.flops / (ev / 3) / FP64_PEAK,
+        "flops_formula": "per term ceil(h/2) - 1 products of the padded 2E x 2E term matrix (8 flops per "
+        "complex multiply-add), h = N/2",
+        "reference_algorithm_equivalent_tflops": wref / (ev / 3) / 1e12,
+        "reference_formula": "per term 8 m^3 max(min(h+1, m) - 2, 0), m = 2 nnz(delta) (charpoly.py:316-318)"}
     b4.close()
     if with_cpu:
         # the same requests on the CPU oracle: over terms (all threads per request) on 128 of them,
@@ -294,9 +325,34 @@ def next_rows_run(with_cpu):
     return out
 
 
-# HBM bytes per launch of the dominant kernel on the n in [34, 40] loop sub-batch of the
-# default stream: FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3, scaled by the copy calibration
-TRAFFIC_PM1S_T5_LOOP = 4.41e6
+def counter_traffic(name):
+    """HBM bytes per launch of a kernel from its committed counter summary
+    (profiles/<name>_counters.md, line ``traffic_bytes_per_launch: X``: FETCH_SIZE and
+    WRITE_SIZE of two rocprofv3 --pmc runs of this benchmark, scaled by a copy kernel's
+    calibration), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", name + "_counters.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def w_rep(job):
+    """The reference's product flops for one prepared repeated-row request: over its terms
+    (mixed-radix digits, _hafnian.py:162-180), 8 m^3 max(min(h+1, m) - 2, 0) with m = 2 nnz(delta)
+    (charpoly.py:316-318)."""
+    e = np.asarray(job.edge_reps, dtype=np.int64)
+    h = (2 * int(e.sum()) + (job.oddloop is not None)) // 2
+    first = (e[0] + 2) // 2 if job.glynn and job.oddloop is None else e[0] + 1
+    grids = np.meshgrid(*([np.arange(first)] + [np.arange(x + 1) for x in e[1:]]), indexing="ij")
+    k = np.stack([g.ravel() for g in grids])
+    delta = 2 * k - e[:, None] if job.glynn else k
+    m = 2 * (delta != 0).sum(axis=0)
+    return float((8.0 * m ** 3 * np.maximum(np.minimum(h + 1, m) - 2, 0)).sum())
 
 
 def mfma_flops_executed(n):
@@ -309,19 +365,104 @@ def mfma_flops_executed(n):
     return 2.0 ** (h - 1) * ((h + 1) // 2 - 1) * per_product
 
 
+def serve_stream(reqs, qps, Server, seed):
+    """Submit ``reqs`` to a fresh server at Poisson arrival times of rate ``qps`` (seeded) and
+    wait for every result: (wall seconds from the first submission to the last result, the
+    server's stats summary, the values in request order)."""
+    arrivals = W.arrivals(len(reqs), qps, seed=seed)
+    with Server(slots=4, slot_pool_mb=16) as srv:
+        futs = []
+        t0 = time.perf_counter()
+        for (A, l), ta in zip(reqs, arrivals):
+            wait = ta - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            futs.append(srv.submit(A, loop=l))
+        vals = [f.result() for f in futs]
+        wall = time.perf_counter() - t0
+        stats = srv.stats.summary()
+    stats["offered_qps"] = qps
+    return wall, stats, np.array(vals)
+
+
+def batched_steps(reqs, R, B, torch, stream, reps=1):
+    """The requests as consecutive resident batches of R (one plan each), run in turn on one
+    stream: (total wall seconds of the runs, values).  Packing and the H2D copy happen before
+    the timed runs."""
+    batches = [B.DeviceBatch([B.prepare(A, loop=l) for A, l in reqs[k:k + R]]) for k in range(0, len(reqs), R)]
+    for b in batches:
+        b.run(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for b in batches:
+            b.run(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    vals = np.concatenate([b.results() for b in batches])
+    for b in batches:
+        b.close()
+    return wall, vals
+
+
+def dominant_kernel(step0, B, torch, stream):
+    """Roofline of the dominant kernel: the largest tile class (n in [34, 40]) of a step's
+    requests, plain and loop launches timed separately (HIP events on the stream they run on),
+    the slower one reported."""
+    roof = None
+    best = None
+    for want_loop in (False, True):
+        idx = [i for i, (A, l) in enumerate(step0) if A.shape[0] >= 34 and l == want_loop]
+        if not idx:
+            continue
+        sub = B.DeviceBatch([B.prepare(*step0[i]) for i in idx])
+        sub.run(stream)
+        t, _ = timed_runs(sub, 1, torch, stream)
+        wref = sum(W.w_glynn(step0[i][0].shape[0]) for i in idx)
+        wexe = sum(mfma_flops_executed(step0[i][0].shape[0]) for i in idx)
+        if best is None or t > best[0]:
+            best = (t, sub.flops, want_loop, len(idx), wref, wexe)
+        sub.close()
+    if best:
+        t, fl, lp, cnt, wref, wexe = best
+        roof = {
+            "bound": "mfma",
+            "achieved": fl / t / 1e12,
+            "peak": FP64_PEAK / 1e12,
+            "unit": "TFLOP/s",
+            "frac": fl / t / FP64_PEAK,
+            "traffic": counter_traffic("pm1s_T5_%s" % ("loop" if lp else "plain")),
+            "kernel": "wh_sieve_pm1s<5,%s> (FP64 matrix cores, symmetric powers, n in [34,40], %d requests of "
+            "the first step)" % ("loop" if lp else "plain", cnt),
+            "flops_formula": "sum over requests of 2^(h-1) * 8 * n(n+1)/2 * n * (ceil(h/2)-1): the upper "
+            "triangle of the symmetric complex n x n products per term (powers up to ceil(h/2)); "
+            "lower-order trace and loop terms not counted",
+            "mfma_executed_tflops": wexe / t / 1e12,
+            "kernel_time_s": t,
+            "reference_algorithm_equivalent_tflops": wref / t / 1e12,
+        }
+
+    return roof
+
+
+def max_rel(a, b):
+    return float(max(abs(x - y) / max(abs(y), 1e-300) for x, y in zip(a, b)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--requests", type=int, default=128)
-    ap.add_argument("--serve-requests", type=int, default=256)
-    ap.add_argument("--nmin", type=int, default=8, help="drop stream requests with n < nmin")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--step-requests", type=int, default=500, help="requests per step")
+    ap.add_argument("--load", type=float, default=1.0,
+                    help="offered rate / the batched capacity measured on the same requests")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percall", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the batched loop hafnian / montrealer lines")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the dominant kernel's launches (for counter runs under rocprofv3)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -343,78 +484,55 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    stream_all = W.config5(count=max(args.requests, args.serve_requests, 512))
-    reqs = [r for r in stream_all[: args.requests] if r[0].shape[0] >= args.nmin]
-    wl = "mixed stream: even n uniform in [8,40], 50% loop, complex128 CN(0,1/n) symmetrised, seed 141"
-    if args.nmin > 8:
-        wl += " (only n >= %d)" % args.nmin
-    preps = [B.prepare(A, loop=l) for A, l in reqs]
-    batch = B.DeviceBatch(preps)
+    R, K = args.step_requests, args.steps
     stream = torch.cuda.Stream()  # default priority; kernels and events share it
+    if args.roofline_only:
+        print(json.dumps({"roofline": dominant_kernel(W.config5(count=R, seed=141), B, torch, stream)}))
+        return
+    warm = W.config5(count=max(args.warmup, 1) * R, seed=142)
+    reqs = W.config5(count=K * R, seed=141)
 
-    for _ in range(args.warmup):
-        batch.run(stream)
+    # warm-up: the warm-up stream as resident batches of R, then served at that capacity; then
+    # one warm-up step submitted at once (closed loop: the served capacity)
+    wwall, _ = batched_steps(warm, R, B, torch, stream)
+    if args.warmup:
+        serve_stream(warm, len(warm) / wwall, Server, seed=142)
+    with Server(slots=4, slot_pool_mb=16) as srv:
+        t0 = time.perf_counter()
+        for f in [srv.submit(A, loop=l) for A, l in warm[:R]]:
+            f.result()
+        served_cap = R / (time.perf_counter() - t0)
+    wstep, _ = batched_steps(warm[:R], R, B, torch, stream)
+
+    # the batched capacity of the timed requests (K resident batches of R): sets the offered rate
+    bwall, bvals = batched_steps(reqs, R, B, torch, stream)
+    capacity = len(reqs) / bwall
+    qps = args.load * capacity
+
     torch.cuda.synchronize()
-    ev_s, wall = timed_runs(batch, args.steps, torch, stream)
-    value = len(reqs) * args.steps / wall
-    res = batch.results()
+    wall, stats, served_vals = serve_stream(reqs, qps, Server, seed=141)
+    torch.cuda.synchronize()
+    value = len(reqs) / wall
+    batched = {"value": capacity, "unit": "hafnians/s", "ms_per_step": bwall / K * 1e3,
+               "served_over_batched": value / capacity,
+               "served_max_rel_diff_vs_batched": max_rel(served_vals, bvals)}
+    # latency at half load on the first two steps
+    half = reqs[:2 * R]
+    _, stats_half, _ = serve_stream(half, 0.5 * capacity, Server, seed=143)
 
-    # dominant kernel: the largest tile class (n in [34, 40]); plain and loop launches timed
-    # separately on the stream they run on, the slower one reported
-    roof = None
-    best = None
-    for want_loop in (False, True):
-        idx = [i for i, (A, l) in enumerate(reqs) if A.shape[0] >= 34 and l == want_loop]
-        if not idx:
-            continue
-        sub = B.DeviceBatch([preps[i] for i in idx])
-        sub.run(stream)
-        t, _ = timed_runs(sub, 1, torch, stream)
-        wref = sum(W.w_glynn(reqs[i][0].shape[0]) for i in idx)
-        wexe = sum(mfma_flops_executed(reqs[i][0].shape[0]) for i in idx)
-        if best is None or t > best[0]:
-            best = (t, sub.flops, want_loop, len(idx), wref, wexe)
-        sub.close()
-    if best:
-        t, fl, lp, cnt, wref, wexe = best
-        roof = {
-            "bound": "mfma",
-            "achieved": fl / t / 1e12,
-            "peak": FP64_PEAK / 1e12,
-            "unit": "TFLOP/s",
-            "frac": fl / t / FP64_PEAK,
-            # HBM bytes per launch of wh_sieve_pm1s<5, loop> on this sub-batch from rocprofv3
-            # FETCH_SIZE x 2 + WRITE_SIZE (copy-calibrated), profiles/pm1s_T5_loop_counters.md
-            "traffic": TRAFFIC_PM1S_T5_LOOP,
-            "kernel": "wh_sieve_pm1s<5,%s> (FP64 matrix cores, symmetric powers, n in [34,40], %d requests)"
-            % ("loop" if lp else "plain", cnt),
-            "flops_formula": "sum over requests of 2^(h-1) * 8 * n(n+1)/2 * n * (ceil(h/2)-1): the upper "
-            "triangle of the symmetric complex n x n products per term (powers up to ceil(h/2)); "
-            "lower-order trace and loop terms not counted",
-            "mfma_executed_tflops": wexe / t / 1e12,
-            "kernel_time_s": t,
-            "reference_algorithm_equivalent_tflops": wref / t / 1e12,
-        }
+    roof = dominant_kernel(reqs[:R], B, torch, stream)
 
     pc = None
     if not args.no_percall:
         rate, vals = per_call(reqs, tw)
-        pc = {
-            "value": rate,
-            "unit": "hafnians/s",
-            "max_rel_diff_vs_batched": float(max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(vals, res))),
-        }
-
-    serving = None
-    if not args.no_serve:
-        sreqs = stream_all[: args.serve_requests]
-        serving = {"requests": len(sreqs)}
-        for load in (0.5, 0.9):
-            serving["load_%.1f" % load] = serve_run(sreqs, load * value, Server)
+        pc = {"value": rate, "unit": "hafnians/s", "served_over_per_call": value / rate,
+              "max_rel_diff_vs_served": max_rel(vals, served_vals)}
+        if batched:
+            batched["batched_over_per_call"] = batched["value"] / rate
 
     small = None
     if not args.no_percall:
-        sm = [r for r in stream_all if r[0].shape[0] <= 24][:256]
+        sm = [r for r in warm + reqs if r[0].shape[0] <= 24][:256]
         sb = B.DeviceBatch([B.prepare(A, loop=l) for A, l in sm])
         sb.run(stream)
         _, w = timed_runs(sb, 3, torch, stream)
@@ -423,40 +541,44 @@ def main():
         sb.run(stream)
         _, wg = timed_runs(sb, 3, torch, stream)
         rate, _ = per_call(sm, tw)
-        served, served_vals = serve_closed(sm, Server)
+        served, served_sm = serve_closed(sm, Server)
         small = {"requests": len(sm), "batched": 3 * len(sm) / w, "per_call": rate,
                  "batched_over_per_call": 3 * len(sm) / w / rate,
                  "batched_hip_graph": 3 * len(sm) / wg,
                  "graph_equals_eager": bool(np.array_equal(sb.results(), r_eager)),
                  "served": served, "served_over_per_call": served / rate,
-                 "served_max_rel_diff_vs_batched": float(max(
-                     abs(a - b) / max(abs(b), 1e-300) for a, b in zip(served_vals, r_eager)))}
+                 "served_max_rel_diff_vs_batched": max_rel(served_sm, r_eager)}
         sb.close()
 
     configs = None if args.no_configs else configs_run(torch, tw, B, stream, with_cpu=not args.no_cpu)
     next_rows = None if args.no_next else next_rows_run(not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(reqs)
-    head = serving["load_0.9"] if serving else {}
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "hafnians/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "ms_per_step_hip_events": ev_s / args.steps * 1e3,
+        "ms_per_step": wall / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": wl, "requests_per_step": len(reqs), "parallelism": "single GPU"},
-        "p50_ms": head.get("p50_ms"),
-        "p99_ms": head.get("p99_ms"),
+        "config": {"workload": "serving stream: %d requests (%d steps of %d), Poisson arrivals at %.1f QPS (%.2f x "
+                   "the batched capacity %.1f/s of the same requests, measured first), even n uniform in [8,40], "
+                   "50%% loop, complex128 CN(0,1/n) symmetrised, seed 141; host preparation and submission "
+                   "inside the timed region" % (len(reqs), K, R, qps, args.load, capacity),
+                   "requests": len(reqs), "step_requests": R, "qps": qps, "parallelism": "single GPU"},
+        "p50_ms": stats.get("p50_ms"),
+        "p99_ms": stats.get("p99_ms"),
+        "serving": stats,
+        "serving_half_load_first_%d" % len(half): stats_half,
+        "warmup_step_served_closed_loop": served_cap,
+        "warmup_step_batched": R / wstep,
+        "batched": batched,
         "per_call": pc,
-        "batched_over_per_call": (value / pc["value"]) if pc else None,
-        "serving": serving,
         "small_stream": small,
         "configs": configs,
         "next_rows": next_rows,

@@ -126,6 +126,16 @@ typedef struct wh_request {
  * the default, forming those powers directly (the same traces; on the GPU the serial
  * reduction costs more than the few extra products). */
 #define WH_OPT_CHARPOLY 32
+/* wh_request.options: run an all-ones or repeated-row request at tile side t (n_pad = 8t) instead
+ * of the smallest side that holds its matrix (more zero padding; t <= 8 on the symmetric
+ * matrix-core kernel, 6 on the vector kernel, 5 on the full-power matrix-core kernel, 2 on the
+ * repeated-row kernel), and with w waves per workgroup (1..4, within the LDS) instead of the
+ * count that fits the most waves per CU.  The batch fails with WH_EINVAL for a value the request's
+ * kernel cannot take.  For the tuner and for tests of the larger tile classes on small inputs. */
+#define WH_OPT_TILE_SHIFT 8
+#define WH_OPT_WAVES_SHIFT 12
+#define WH_OPT_TILE(t) ((t) << WH_OPT_TILE_SHIFT)
+#define WH_OPT_WAVES(w) ((w) << WH_OPT_WAVES_SHIFT)
 /* wh_request.options: a montrealer request (montrealer / lmontrealer, _montrealer.py:37-101)
  * over n = E modes.  a_off: X Sigma (2E x 2E, rows of Sigma swapped by halves); for the loop
  * montrealer d_off: X zeta and v_off: conj(X zeta), else both -1.  edge_reps, glynn and
@@ -290,6 +300,17 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
 
 /* The table entry, or -1 for an entry out of range. */
 int64_t wh_tune_get(int kind, int T, int loop);
+
+/* How plans and the serving engine launch a batch's complex all-ones requests with n in [26, 48]
+ * (the symmetric matrix-core kernel's tile sides 4..6): 0 = one launch per kernel class, then the
+ * fold; 1 (default) = one persistent launch for all of them (workgroups take the classes' term
+ * chunks from a device queue, largest class first, and the last workgroup folds every request of
+ * the batch) when the batch has two such classes or more; 2 = persistent whenever it has one.
+ * The batch's other classes have their own launches first.  Results are bit-identical either
+ * way (the same chunks, the same per-chunk work, the same fold order).  No counterpart in the
+ * reference (numba runs one prange per call).  Returns the previous mode; mode < 0 only returns
+ * the current one. */
+int wh_tune_persistent(int mode);
 
 #ifdef __cplusplus
 }

@@ -401,3 +401,67 @@ def hafnian_batched(A, cutoff, mu=None, renorm=False, precision="double"):
     n = A.shape[0]
     mu = np.zeros(n, dtype=complex) if mu is None else np.asarray(mu)
     return hermite_recursion(-A, mu, [cutoff] * n, 1.0, renorm, precision)
+
+
+# ---- timed CPU baseline (cpu_timing.cpp): the reference's CPU path shaped as it runs there
+# (per-term products as a blocked complex product, FMA on, OpenMP over terms); bench.py's
+# cpu_baseline leg times it, the tests check it against the restatement above.
+_CT = None
+
+
+def _host_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("flags"):
+                    return " avx512f " in ln + " "
+    except OSError:
+        pass
+    return False
+
+
+def cpu_timing_lib():
+    """The AVX-512 build where the host supports it, else the AVX2 one: (library, isa name)."""
+    global _CT
+    if _CT is None:
+        isa = "x86-64-v4" if _host_has_avx512() else "x86-64-v3"
+        path = os.path.join(_HERE, "libcpu_timing_%s.so" % ("v4" if isa == "x86-64-v4" else "v3"))
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.ct_sieve.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, dp]
+        L.ct_sieve_many.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        _CT = (L, isa)
+    return _CT
+
+
+def ct_prepare(A, loop):
+    """The reordered matrix (and loop weights) of an all-ones request (matched_reps with every
+    repetition one), as the timed baseline takes them."""
+    n = A.shape[0]
+    x, _, _ = pair_modes([1] * n)
+    Ax = _c(A[np.ix_(x, x)])
+    Dx = _c(np.diag(A)[x]) if loop else None
+    return Ax, Dx
+
+
+def ct_sieve(Ax, Dx, t0, t1, nthreads=0):
+    """Unscaled sum of the Glynn terms [t0, t1) of one prepared all-ones request."""
+    L, _ = cpu_timing_lib()
+    out = np.zeros(2)
+    L.ct_sieve(_dp(Ax.view(np.float64)), None if Dx is None else _dp(Dx.view(np.float64)), Ax.shape[0],
+               int(t0), int(t1), int(nthreads), _dp(out))
+    return complex(out[0], out[1])
+
+
+def ct_sieve_many(Axs, Dxs, nthreads=0):
+    """Hafnians of prepared all-ones requests of one size, one request per thread."""
+    L, _ = cpu_timing_lib()
+    n = Axs[0].shape[0]
+    A = np.ascontiguousarray(np.stack(Axs), dtype=np.complex128)
+    D = None if Dxs is None else np.ascontiguousarray(np.stack(Dxs), dtype=np.complex128)
+    out = np.zeros(2 * len(Axs))
+    L.ct_sieve_many(_dp(A.view(np.float64).reshape(-1)), None if D is None else _dp(D.view(np.float64).reshape(-1)),
+                    n, len(Axs), int(nthreads), _dp(out))
+    return out.view(np.complex128)

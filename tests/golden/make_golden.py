@@ -3,7 +3,7 @@
     python tests/golden/make_golden.py small   # seconds: configs 1, 2, 4, 5 samples
     python tests/golden/make_golden.py n40     # minutes: config 3 and structured n = 40
     python tests/golden/make_golden.py band    # ~40 min: random n = 30..38, structured 32..44
-    python tests/golden/make_golden.py n48     # ~2 h: structured n = 48 (FP64 reference values)
+    python tests/golden/make_golden.py band-update s42_plain_real   # one structured case again
 
 Truth values come from the oracle's double-double restatement (precision="dd"); for n = 40
 the per-term characteristic-polynomial route (variant=1) keeps the cost to minutes.  The
@@ -33,11 +33,13 @@ def _c(z):
 def structured(n_half, loop, seed, real=False):
     """A = W P (A1 (+) A2) P^T W: lhaf(A) = lhaf(A1) lhaf(A2) prod(w).  Off-diagonal entries
     scale by w_i w_j, loop weights by w_i (each vertex is covered once).  ``real`` draws real
-    blocks and real w (signs and magnitudes) and returns a float64 matrix."""
+    blocks and real w (signs and magnitudes) and returns a float64 matrix.  ``n_half`` is the
+    common block size, or a pair of sizes (even sizes for a plain hafnian: an odd block's is 0)."""
     rng = np.random.default_rng(seed)
-    s = np.sqrt(1.0 / (4 * n_half))
+    sizes = tuple(n_half) if isinstance(n_half, (tuple, list)) else (n_half, n_half)
+    s = np.sqrt(1.0 / (2 * sum(sizes)))
     blocks = []
-    for _ in range(2):
+    for n_half in sizes:
         if real:
             B = rng.normal(scale=2 * s, size=(n_half, n_half))
         else:
@@ -49,9 +51,9 @@ def structured(n_half, loop, seed, real=False):
             else:
                 X[np.diag_indices(n_half)] = (rng.normal(size=n_half) + 1j * rng.normal(size=n_half)) / np.sqrt(2)
         blocks.append(X)
-    n = 2 * n_half
+    n = sum(sizes)
     A = np.zeros((n, n), dtype=np.float64 if real else np.complex128)
-    A[:n_half, :n_half], A[n_half:, n_half:] = blocks
+    A[:sizes[0], :sizes[0]], A[sizes[0]:, sizes[0]:] = blocks
     perm = rng.permutation(n)
     A = A[np.ix_(perm, perm)]
     if real:
@@ -145,7 +147,8 @@ def band():
         print(key, "%.0fs" % (time.time() - t0), flush=True)
     for n, real in ((32, False), (36, False), (44, False), (42, True)):
         for lp in (False, True):
-            S, t = structured(n // 2, lp, 4000 + 2 * n + lp, real)
+            halves = n // 2 if lp or (n // 2) % 2 == 0 else (n // 2 - 1, n // 2 + 1)
+            S, t = structured(halves, lp, 4000 + 2 * n + lp, real)
             key = "s%d_%s_%s" % (n, "loop" if lp else "plain", "real" if real else "cplx")
             out[key + "_A"] = S
             out[key + "_truth"] = t
@@ -154,18 +157,20 @@ def band():
     np.savez_compressed(os.path.join(HERE, "golden_band.npz"), **out)
 
 
-def n48():
-    """Structured n = 48 (plain and loop, complex): exact truth, FP64 reference values."""
-    out = {}
-    t0 = time.time()
-    for lp in (False, True):
-        S, t = structured(24, lp, 4096 + lp)
-        key = "s48_%s_cplx" % ("loop" if lp else "plain")
-        out[key + "_A"] = S
-        out[key + "_truth"] = t
-        out[key + "_ref64"] = _c(O.hafnian_sieve(S, loop=lp, precision="double"))
-        print(key, "%.0fs" % (time.time() - t0), flush=True)
-        np.savez_compressed(os.path.join(HERE, "golden_n48.npz"), **out)
+def band_update(key):
+    """Regenerates one structured case of golden_band.npz (e.g. s42_plain_real) in place."""
+    path = os.path.join(HERE, "golden_band.npz")
+    with np.load(path, allow_pickle=False) as g:
+        out = {k: g[k] for k in g.files}
+    n = int(key[1:key.index("_")])
+    lp = "_loop_" in key
+    real = key.endswith("_real")
+    halves = n // 2 if lp or (n // 2) % 2 == 0 else (n // 2 - 1, n // 2 + 1)
+    S, t = structured(halves, lp, 4000 + 2 * n + lp, real)
+    out[key + "_A"] = S
+    out[key + "_truth"] = t
+    out[key + "_ref64"] = _c(O.hafnian_sieve(S, loop=lp, precision="double"))
+    np.savez_compressed(path, **out)
 
 
 if __name__ == "__main__":
@@ -176,5 +181,5 @@ if __name__ == "__main__":
         n40()
     if what in ("band", "all"):
         band()
-    if what in ("n48", "all"):
-        n48()
+    if what == "band-update":
+        band_update(sys.argv[2])

@@ -159,3 +159,35 @@ def test_native_preparation_declines_other_requests():
                     (np.zeros((66, 66)), False)):
         k, _, _ = _prep_native(L, np.ascontiguousarray(A), loop)
         assert k == 0
+
+
+def test_layout_refuses_chunk_counts_beyond_the_launch_limit():
+    """A repeated-row request with about 6e11 terms needs more than 2^31 - 1 general-kernel
+    chunks of at most 256 terms: the layout fails with WH_EUNSUPPORTED instead of truncating the
+    chunk count (host only)."""
+    L = _native.load()
+    r = _native.WhRequest()
+    r.E = 24
+    r.glynn = 1
+    r.a_off = 0
+    r.d_off = r.v_off = -1
+    reps = [2] * 20 + [3] * 4  # 2 * 3^19 * 4^4 = 5.95e11 Glynn terms, N = 104 photons
+    for i, e in enumerate(reps):
+        r.edge_reps[i] = e
+    b = ctypes.c_void_p()
+    rc = L.wh_batch_create(ctypes.byref(r), 1, 0, ctypes.byref(b))
+    assert rc == -4 and b"launch limit" in L.wh_last_error()
+    slots = ctypes.c_int64()
+    assert L.wh_request_partials(ctypes.byref(r), 0, ctypes.byref(slots)) == -4
+
+
+def test_persistent_mode_round_trip():
+    L = _native.load()
+    prev = L.wh_tune_persistent(-1)
+    try:
+        assert prev in (0, 1, 2)
+        assert L.wh_tune_persistent(0) == prev
+        assert L.wh_tune_persistent(-1) == 0
+        assert L.wh_tune_persistent(3) == -1
+    finally:
+        L.wh_tune_persistent(prev)

@@ -102,8 +102,8 @@ def test_structured_loop_batch():
     """Structured loop matrices with exact truth, batched; bar: 1e-10 or twice the reference
     algorithm's own FP64 error on the same input (the oracle's faithful restatement)."""
     mats, truths = [], []
-    for s in range(4):
-        A, t = _structured(12, True, 100 + s)
+    for s, half in enumerate((12, 12, 16, 16)):  # n = 24 and n = 32 (tile sides 3 and 4)
+        A, t = _structured(half, True, 100 + s)
         mats.append(A)
         truths.append(t)
     res = B.hafnian_batch(mats, loop=True)
@@ -135,34 +135,59 @@ def test_half_and_full_products(loop, real):
         assert relerr(f, truth) < bar, ("full", A.shape[0], relerr(f, truth), bar)
 
 
+def _band_cases(loop):
+    """The n = 30..44 complex fixtures (random n = 30..38, structured direct sums n = 32, 36, 44)
+    of tests/golden/golden_band.npz: (key, A, truth, FP64 reference value)."""
+    import os
+
+    out = []
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name in ("golden_band.npz",):
+        path = os.path.join(here, name)
+        if not os.path.exists(path):
+            continue
+        g = np.load(path, allow_pickle=False)
+        for key in sorted(k[:-2] for k in g.files if k.endswith("_A")):
+            if key.endswith("_cplx") and ("_loop_" in key) == loop:
+                out.append((key, g[key + "_A"], g[key + "_truth"], g[key + "_ref64"]))
+    return out
+
+
 @pytest.mark.parametrize("loop", [False, True])
 def test_matrix_cores_and_vector_units(loop):
     """The complex all-ones kernel on the FP64 matrix cores with symmetric powers (default), with
-    full powers, and on the vector units, tile sides 1..6 (n = 8 .. 48), against truth; bar as
-    above."""
+    full powers, and on the vector units, tile sides 1..6, against truth: n = 4 .. 24 from the
+    oracle's double-double restatement, n = 30 .. 44 from the committed fixtures (random inputs
+    with double-double truth, structured direct sums with exact truth); bar as above.  n = 48
+    against closed forms in test_gpu_parity.py."""
     rng = np.random.default_rng(61 + loop)
-    sizes = [4, 6, 8, 12, 16, 20, 24, 30, 32, 36, 40, 44, 48] if not loop else [6, 8, 14, 16, 22, 24, 30, 32, 38, 40, 46]
-    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in sizes]
-    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    sizes = [4, 6, 8, 12, 16, 20, 24] if not loop else [6, 8, 14, 16, 22, 24]
+    cases = []
+    for n in sizes:
+        A = random_symmetric(rng, n, scale=1 / np.sqrt(n))
+        truth = O.hafnian_sieve(A, loop=loop, precision="dd")
+        ref64 = O.hafnian_sieve(A, loop=loop, precision="double")
+        cases.append(("random n=%d" % n, A, truth, ref64))
+    band = _band_cases(loop)
+    assert len(band) >= 7, "golden_band.npz missing"
+    cases += band
+    preps = [B.prepare(A, loop=loop) for _, A, _, _ in cases]
     runs = {"symmetric powers": B.DeviceBatch(preps), "full powers": B.DeviceBatch(preps, full_powers=True),
             "vector units": B.DeviceBatch(preps, valu_products=True)}
     res = {}
     for name, db in runs.items():
-        print("all-ones kernel: %s, loop=%s" % (name, loop), flush=True)
+        print("all-ones kernel: %s, loop=%s, %d requests" % (name, loop, len(cases)), flush=True)
         db.run()
         res[name] = db.results()
-    for k, (A, l) in enumerate(reqs):
-        n = A.shape[0]
-        if n <= 24:
-            truth = O.hafnian_sieve(A, loop=l, precision="dd")
-            ref64 = O.hafnian_sieve(A, loop=l, precision="double")
-            bar = max(TOL, 2 * relerr(ref64, truth))
-            for name in runs:
-                assert relerr(res[name][k], truth) < bar, (name, n, relerr(res[name][k], truth), bar)
-        else:  # beyond the oracle's seconds: the kernels agree to the bar
-            v = res["vector units"][k]
-            for name in ("symmetric powers", "full powers"):
-                assert relerr(res[name][k], v) < 1e-10, (name, n, relerr(res[name][k], v))
+    for k, (key, A, truth, ref64) in enumerate(cases):
+        ref_err = relerr(ref64, truth)
+        for name in runs:
+            # the default formulation to the documented bar; the two cross-check formulations
+            # round differently, and where the sieve's cancellation lifts the reference
+            # algorithm's own FP64 error far above 1e-10 (the structured n = 44 loop case:
+            # 7.6e-7) they are held to four times that error
+            bar = max(TOL, (2 if name == "symmetric powers" else 4) * ref_err)
+            assert relerr(res[name][k], truth) < bar, (name, key, relerr(res[name][k], truth), bar)
 
 
 def test_graph_replay_equals_eager():
@@ -268,8 +293,9 @@ def test_odd_chunk_sizes_agree(loop):
 @pytest.mark.parametrize("loop", [False, True])
 def test_direct_powers_and_charpoly_routes(loop):
     """Repeated-row requests whose term matrices are smaller than h + 1 (so the reference extends
-    the power traces by the characteristic polynomial): the general kernel's default (every
-    power formed directly) and the reference's route (WH_OPT_CHARPOLY) both against truth; bar:
+    the power traces by the characteristic polynomial): the default route (the repeated-row
+    kernel where it applies), the general kernel forming every power directly, and the
+    reference's route (WH_OPT_CHARPOLY), all against truth; bar:
     1e-10 or twice the reference algorithm's own FP64 error on the input.  Odd photon numbers
     included for loops."""
     rng = np.random.default_rng(77 + loop)
@@ -283,15 +309,136 @@ def test_direct_powers_and_charpoly_routes(loop):
         mu = (rng.normal(size=n) + 1j * rng.normal(size=n)) if loop else None
         reqs.append((A, rpt, mu))
     preps = [B.prepare_repeated(A, rpt, mu=mu, loop=loop) for A, rpt, mu in reqs]
-    direct = B.DeviceBatch(preps)
+    default = B.DeviceBatch(preps)  # even N, 2E <= 16, h <= 16: the repeated-row kernel
+    direct = B.DeviceBatch(preps, force_generic=True)
     charpoly = B.DeviceBatch(preps, charpoly=True)
-    direct.run()
-    charpoly.run()
-    rd, rc = direct.results(), charpoly.results()
+    for db in (default, direct, charpoly):
+        db.run()
+    rr, rd, rc = default.results(), direct.results(), charpoly.results()
     assert direct.flops > charpoly.flops  # the direct route forms more (small) products
-    for (A, rpt, mu), d, c in zip(reqs, rd, rc):
+    for (A, rpt, mu), r, d, c in zip(reqs, rr, rd, rc):
         truth = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, precision="dd")
         ref64 = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, precision="double")
         bar = max(TOL, 2 * relerr(ref64, truth))
+        assert relerr(r, truth) < bar, ("default", rpt, relerr(r, truth), bar)
         assert relerr(d, truth) < bar, ("direct", rpt, relerr(d, truth), bar)
         assert relerr(c, truth) < bar, ("charpoly", rpt, relerr(c, truth), bar)
+
+
+REP_CASES = [
+    ([2, 1, 0, 1], False),            # 2E = 6: T = 1, h = 2
+    ([3, 2, 2, 1], True),             # h = 4
+    ([2, 2, 2, 2, 2, 2], False),      # 2E = 12: T = 2, h = 6
+    ([5, 1, 0, 2, 3, 1, 2, 2], True),  # h = 8
+    ([4, 3, 5, 2, 6, 4, 3, 5], True),  # h = 16: coefficient order cap 16
+    ([1, 1, 3, 1, 1, 1, 1, 1], False),  # mostly single photons
+]
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("glynn", [True, False])
+def test_repeated_row_kernel_matches_generic_and_truth(real, glynn):
+    """Repeated-row requests (even N, 2E <= 16, h <= 16) run on the register-tiled kernel with
+    column factors delta, against the double-double oracle; the general kernel (forced) too for
+    the Glynn sieve (inclusion-exclusion with multiplicities up to 6 cancels to 1e-8 in FP64,
+    where the general kernel's direct powers round differently from the reference's route; its
+    inclusion-exclusion is checked on milder inputs in test_inclexcl_vs_truth)."""
+    rng = np.random.default_rng(70 + 2 * real + glynn)
+    preps, reqs = [], []
+    for rpt, loop in REP_CASES:
+        n = len(rpt)
+        A = random_symmetric(rng, n, complex_=not real, scale=1 / np.sqrt(n))
+        mu = rng.normal(size=n) + (0 if real else 1j * rng.normal(size=n))
+        preps.append(B.prepare_repeated(A, rpt, mu=mu, loop=loop, glynn=glynn))
+        reqs.append((A, rpt, mu, loop))
+    fast = B.DeviceBatch(preps)
+    gen = B.DeviceBatch(preps, force_generic=True)
+    fast.run()
+    gen.run()
+    rf, rg = fast.results(), gen.results()
+    for (A, rpt, mu, loop), a, g in zip(reqs, rf, rg):
+        truth = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, glynn=glynn, precision="dd")
+        ref64 = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, glynn=glynn, precision="double")
+        bar = max(TOL, 2 * relerr(ref64, truth))
+        print("rep rpt=%s loop=%s glynn=%s real=%s: %.2e %.2e (bar %.2e)"
+              % (rpt, loop, glynn, real, relerr(a, truth), relerr(g, truth), bar), flush=True)
+        assert relerr(a, truth) < bar, (rpt, loop, a, truth)
+        if glynn:
+            assert relerr(g, truth) < bar, (rpt, loop, g, truth)
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_forced_tile_sides_and_waves(loop):
+    """The symmetric matrix-core kernel at every tile side 1..8 (the sides 7 and 8 serve
+    n = 50..64) on small inputs padded to that side, and the vector and repeated-row kernels
+    with forced waves per workgroup, against truth."""
+    rng = np.random.default_rng(91 + loop)
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in (8, 14, 22)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    truths = [O.hafnian_sieve(A, loop=l, precision="dd") for A, l in reqs]
+    runs = [dict(tile=t) for t in range(3, 9)] + [dict(tile=6, waves=w) for w in (1, 2, 3)] + \
+        [dict(valu_products=True, waves=w) for w in (1, 2, 4)]
+    for kw in runs:
+        print("forced %s loop=%s" % (kw, loop), flush=True)
+        db = B.DeviceBatch(preps, **kw)
+        db.run()
+        res = db.results()
+        db.close()
+        for (A, l), r, t in zip(reqs, res, truths):
+            assert relerr(r, t) < TOL, (kw, A.shape[0], relerr(r, t))
+    rpt = [3, 2, 2, 1, 2, 2, 1, 1]
+    A = random_symmetric(rng, 8, scale=0.35)
+    mu = rng.normal(size=8) + 1j * rng.normal(size=8)
+    t = O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, precision="dd")
+    for w in (1, 2, 3, 4):
+        db = B.DeviceBatch([B.prepare_repeated(A, rpt, mu=mu, loop=loop)], waves=w)
+        db.run()
+        assert relerr(db.results()[0], t) < TOL, ("repeated-row kernel", w)
+        db.close()
+
+
+def test_tile_and_waves_that_do_not_fit_are_refused():
+    rng = np.random.default_rng(93)
+    A = random_symmetric(rng, 20)
+    for kw in (dict(tile=2), dict(tile=9), dict(valu_products=True, tile=7), dict(waves=5)):
+        with pytest.raises(ValueError, match="tile side or waves"):
+            B.DeviceBatch([B.prepare(A)], **kw)
+
+
+def test_persistent_launch_equals_per_class_launches():
+    """A batch whose complex all-ones requests span every tile side 1..6, plain and loop (one
+    persistent launch: the classes' chunks from one device queue, the fold by the last
+    workgroup), beside a real request and a repeated-row request (their own launches first):
+    bit-identical to one launch per class, on repeated runs of the same plan, and against truth."""
+    from thewalrus_amd import _native
+
+    L = _native.load()
+    rng = np.random.default_rng(97)
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), lp)
+            for n in (6, 8, 12, 16, 20, 24, 30, 32, 36, 40, 42) for lp in (False, True)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    preps.append(B.prepare(random_symmetric(rng, 20, complex_=False, scale=0.3)))
+    Ar = random_symmetric(rng, 6, scale=0.4)
+    preps.append(B.prepare_repeated(Ar, [3, 2, 1, 2, 2, 2], mu=np.diag(Ar).copy(), loop=True))
+    prev = L.wh_tune_persistent(-1)
+    try:
+        res = {}
+        for mode in (0, 2, 1):
+            assert L.wh_tune_persistent(mode) >= 0
+            db = B.DeviceBatch(preps)
+            runs = []
+            for _ in range(2):
+                db.out.zero_()
+                db.run()
+                runs.append(db.results())
+            db.close()
+            assert np.array_equal(runs[0], runs[1]), mode
+            res[mode] = runs[0]
+            print("persistent mode %d done" % mode, flush=True)
+    finally:
+        L.wh_tune_persistent(prev)
+    assert np.array_equal(res[0], res[2]) and np.array_equal(res[0], res[1])
+    for (A, l), r in zip(reqs, res[2]):
+        if A.shape[0] <= 24:
+            truth = O.hafnian_sieve(A, loop=l, precision="dd")
+            assert relerr(r, truth) < TOL, (A.shape[0], l)

@@ -95,3 +95,38 @@ def test_structured_n40(kind):
     err, ref_err = relerr(got, t), relerr(g["s40_%s_ref64" % kind], t)
     print("structured n=40 %s: device err %.3e, reference-algorithm FP64 err %.3e" % (kind, err, ref_err))
     assert err < _bar(t, g["s40_%s_ref64" % kind])
+
+
+def _band_keys():
+    path = os.path.join(HERE, "golden_band.npz")
+    if not os.path.exists(path):
+        return []
+    with np.load(path, allow_pickle=False) as g:
+        return sorted(k[:-2] for k in g.files if k.endswith("_A"))
+
+
+@pytest.mark.parametrize("key", _band_keys() or ["missing"])
+def test_band_n30_to_48(key):
+    """Tile classes T = 4..6 (n = 26..48) against truth: random n = 30..38 (complex, and real
+    n = 30, 34) with double-double truth, structured direct sums n = 32, 36, 44 (complex) and
+    42 (real) with exact truth; per call and inside one batch with the rest of its class."""
+    g = _load("golden_band.npz")
+    if key == "missing":
+        pytest.fail("golden_band.npz has no cases")
+    A, t, r64 = g[key + "_A"], g[key + "_truth"], g[key + "_ref64"]
+    loop = "_loop_" in key
+    got = tw.hafnian(A, loop=loop)
+    err = relerr(got, t)
+    print("%s: device err %.3e, reference-algorithm FP64 err %.3e" % (key, err, relerr(r64, t)), flush=True)
+    assert err < _bar(t, r64)
+
+
+def test_band_batched_equals_per_call():
+    """The band's requests in one mixed batch: the same bits as per call."""
+    g = _load("golden_band.npz")
+    keys = _band_keys()
+    mats = [g[k + "_A"] for k in keys]
+    loops = [("_loop_" in k) for k in keys]
+    res = B.hafnian_batch(mats, loop=loops)
+    for k, A, lp, r in zip(keys, mats, loops, res):
+        assert complex(tw.hafnian(A, loop=lp)) == complex(r), k

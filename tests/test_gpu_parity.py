@@ -55,13 +55,30 @@ def test_random_complex_vs_truth(n, loop):
     assert relerr(got, truth) < TOL, (got, truth, ref64)
 
 
-@pytest.mark.parametrize("n", [5, 7, 9, 11, 13])
+@pytest.mark.parametrize("n", [5, 7, 9, 11, 13, 17, 19, 21])
 def test_odd_loop_vs_truth(n):
     rng = np.random.default_rng(2000 + n)
     A = random_symmetric(rng, n, scale=1.0 / np.sqrt(n))
     got = tw.hafnian(A, loop=True)
     truth = O.hafnian_sieve(A, loop=True, precision="dd")
     assert relerr(got, truth) < TOL
+
+
+@pytest.mark.parametrize("rpt", [[1] * 17 + [2], [2, 1, 1, 3, 1, 1, 2, 1, 1, 1, 1, 2, 1, 1]])
+def test_odd_repeated_loop_vs_truth(rpt):
+    """Odd photon numbers whose reordered side 2E exceeds 16: the general kernel's path that
+    forms powers out of place, with the odd-N loop scalars as bilinear sums over them."""
+    n = len(rpt)
+    rng = np.random.default_rng(2100 + n)
+    A = random_symmetric(rng, n, scale=1.0 / np.sqrt(2 * n))
+    mu = rng.normal(size=n) + 1j * rng.normal(size=n)
+    assert sum(rpt) % 2 == 1
+    print("odd repeated loop rpt=%s" % rpt, flush=True)
+    got = tw.hafnian_repeated(A, rpt, mu=mu, loop=True)
+    truth = O.hafnian_sieve(A, loop=True, rpt=rpt, mu=mu, precision="dd")
+    ref64 = O.hafnian_sieve(A, loop=True, rpt=rpt, mu=mu, precision="double")
+    bar = max(TOL, 2 * relerr(ref64, truth))
+    assert relerr(got, truth) < bar, (got, truth, ref64)
 
 
 @pytest.mark.parametrize("n", [6, 8, 12])
@@ -127,3 +144,53 @@ def test_seam_matches_oracle_seam():
     got = _native.calc_hafnian(Ax, e, True)
     ref = O.calc_sieve(Ax, e, glynn=True, precision="double")
     assert relerr(got, ref) < 1e-12
+
+
+def _involutions(n):
+    """Telephone numbers T(n): perfect matchings with loops of K_n, the loop hafnian of J_n."""
+    a, b = 1, 1
+    for k in range(2, n + 1):
+        a, b = b, b + (k - 1) * a
+    return b if n >= 1 else 1
+
+
+def _dfact(n):
+    r = 1
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+@pytest.mark.parametrize("n", [46, 48, 50])
+@pytest.mark.parametrize("loop", [False, True])
+def test_rank_one_closed_forms(n, loop):
+    """n = 46, 48 (tile side 6) and 50 (tile side 7, past the earlier limit of 48):
+    haf(x x^T) = (n-1)!! prod x and lhaf(x x^T with diagonal x) = T(n) prod x, exact closed forms;
+    the sieve barely cancels on these (sum |term| / |sum| about 3 at n = 28)."""
+    rng = np.random.default_rng(5000 + n + loop)
+    x = np.exp(1j * rng.uniform(0, 2 * np.pi, n)) * rng.uniform(0.9, 1.1, n)
+    A = np.outer(x, x)
+    if loop:
+        A[np.diag_indices(n)] = x
+    print("n=%d rank one loop=%s" % (n, loop), flush=True)
+    got = tw.hafnian(A, loop=loop)
+    want = complex(np.prod(x)) * float(_involutions(n) if loop else _dfact(n - 1))
+    assert relerr(got, want) < TOL, (got, want)
+
+
+def test_n50_direct_sum_of_rank_ones():
+    """n = 50 as the direct sum of rank-one blocks of 24 and 26 modes, permuted: the product of
+    the two closed forms (the term matrices' powers have rank two)."""
+    rng = np.random.default_rng(5002)
+    x1 = np.exp(1j * rng.uniform(0, 2 * np.pi, 24)) * rng.uniform(0.9, 1.1, 24)
+    x2 = np.exp(1j * rng.uniform(0, 2 * np.pi, 26)) * rng.uniform(0.9, 1.1, 26)
+    A = np.zeros((50, 50), dtype=np.complex128)
+    A[:24, :24] = np.outer(x1, x1)
+    A[24:, 24:] = np.outer(x2, x2)
+    p = rng.permutation(50)
+    A = A[np.ix_(p, p)]
+    print("n=50 direct sum of rank ones", flush=True)
+    got = tw.hafnian(A)
+    want = complex(np.prod(x1) * np.prod(x2)) * float(_dfact(23) * _dfact(25))
+    assert relerr(got, want) < TOL, (got, want)

@@ -109,8 +109,9 @@ def test_gpu_loop_hafnian_batch_real_inputs():
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,fixed,N_cutoff,nD", [(5, [1, 0, 2, 1], 1, 10), (4, [1, 1, 1], 1, 10), (6, [2, 1, 0, 1, 1], 6, 4)])
 def test_gpu_loop_hafnian_batch_gamma(n, fixed, N_cutoff, nD):
-    """Row k equals the single-diagonal batch with D[k], bit for bit (same chunks, same per-row
-    arithmetic), and the oracle to the bar."""
+    """Row k equals the single-diagonal batch with D[k], bit for bit, and the oracle to the bar.
+    These calls have fewer than 1024 term chunks, so each diagonal gets its own workgroups and
+    the same per-row arithmetic as the single-diagonal call."""
     rng = np.random.default_rng(900 + n + nD)
     A = random_symmetric(rng, n, scale=0.5)
     D = rng.normal(size=(nD, n)) + 1j * rng.normal(size=(nD, n))
@@ -133,3 +134,21 @@ def test_gpu_loop_hafnian_batch_many_chunks():
     fixed = [3, 2, 2, 1, 3, 2, 1, 2]   # 2 x 16 = 32 fixed photons
     print("lhb many chunks", flush=True)
     _check(loop_hafnian_batch(A, D, fixed, 10), A, D, fixed, 10)
+
+
+@pytest.mark.gpu
+def test_gpu_loop_hafnian_batch_gamma_many_chunks():
+    """More than 1024 term chunks: each workgroup then serves all nD diagonals of its chunk
+    (the row and column chains per diagonal), a different path from the few-chunk split; every
+    row is checked against the oracle's single-diagonal batch to the bar."""
+    rng = np.random.default_rng(32)
+    n = 10
+    A = random_symmetric(rng, n, scale=0.35)
+    nD = 3
+    D = rng.normal(size=(nD, n)) + 1j * rng.normal(size=(nD, n))
+    fixed = [3, 3, 2, 2, 3, 2, 2, 2, 1]  # edge multiplicities [3, 2, 2, 2, 1]: 6 x 216 = 1296 terms
+    print("lhb gamma many chunks", flush=True)
+    G = loop_hafnian_batch_gamma(A, D, fixed, 10)
+    assert G.shape == (nD, 11)
+    for k in range(nD):
+        _check(G[k], A, D[k], fixed, 10)

@@ -156,10 +156,13 @@ class DeviceBatch:
         charpoly (bool): on the general kernel, take the traces above the term matrix's side
             from the characteristic polynomial as the reference does, instead of forming those
             powers directly
+        tile (int): run the all-ones and repeated-row kernels at this tile side (n_pad = 8 tile)
+            instead of the smallest that holds each matrix (0 = automatic)
+        waves (int): waves per workgroup of those kernels (0 = automatic)
     """
 
     def __init__(self, preps, device=0, chunk_terms=0, force_generic=False, full_products=False,
-                 valu_products=False, full_powers=False, charpoly=False):
+                 valu_products=False, full_powers=False, charpoly=False, tile=0, waves=0):
         import torch
 
         self.torch = torch
@@ -182,7 +185,7 @@ class DeviceBatch:
         if self.nreq == 0:
             return
         options = ((1 if force_generic else 0) | (2 if full_products else 0) | (8 if valu_products else 0)
-                   | (16 if full_powers else 0) | (32 if charpoly else 0))
+                   | (16 if full_powers else 0) | (32 if charpoly else 0) | (int(tile) << 8) | (int(waves) << 12))
         reqs, pool = pack_jobs(jobs, options=options)
         with torch.cuda.device(self.device):
             self.pool = torch.from_numpy(pool.view(np.float64).copy()).to(self.device)

@@ -45,6 +45,7 @@ int fail(int code, const char* fmt, ...) {
             return fail(WH_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));            \
     } while (0)
 
+constexpr int64_t kMaxBlocks = 0x7fffffff;  // workgroups of one launch (grid x dimension)
 constexpr int kGenericMaxSide = 48;  // LDS holds three 48 x 48 complex matrices per wave
 constexpr int kMtlMaxModes = 24;     // montrealer on the all-ones kernel: side 2n <= 8 * 6
 
@@ -173,23 +174,33 @@ int describe(const wh_request& r, wh_job_dev& j) {
 // Kernel families.  The all-ones kernels take Glynn terms with even N and every multiplicity
 // 1 (m = n for every term), n_pad = 8T: PM1 the register-tiled vector kernel (T x T tiles per
 // lane), PM1S the symmetric-power matrix-core kernel, PM1M the full-power matrix-core kernel.
-// GENERIC: everything else.
-enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2, KIND_PM1S = 3 };
+// REP: repeated rows (any multiplicities) with even N, 2E <= 16 and h <= 16, on the vector
+// kernel's register-tiled products with column factors delta (the all-ones kernels' +-1 in
+// general).  GENERIC: everything else.
+enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2, KIND_PM1S = 3, KIND_REP = 4 };
 static bool all_ones_kind(int k) { return k == KIND_PM1 || k == KIND_PM1M || k == KIND_PM1S; }
-constexpr int kPm1MaxT = 6;
+constexpr int kPm1MaxT = 6;   // tile sides of the vector kernel and of the tuning table
+constexpr int kPm1sMaxT = 8;  // the symmetric matrix-core kernel also takes n in [50, 64]
+constexpr int kRepMaxT = 2;   // REP: 2E <= 16
+constexpr int kRepMaxH = 16;  // REP: h = N / 2 <= 16
 
 struct GroupKey {
     int kind, T, real, loop, half, mtl;
+    int H = 0;   // REP: coefficient order cap (8 or 16)
+    int nw = 0;  // waves per workgroup asked for by WH_OPT_WAVES (0: the LDS-based choice)
     bool operator<(const GroupKey& o) const {
         if (kind != o.kind) return kind < o.kind;
         if (T != o.T) return T < o.T;
         if (real != o.real) return real < o.real;
         if (loop != o.loop) return loop < o.loop;
         if (half != o.half) return half < o.half;
-        return mtl < o.mtl;
+        if (mtl != o.mtl) return mtl < o.mtl;
+        if (H != o.H) return H < o.H;
+        return nw < o.nw;
     }
     bool operator==(const GroupKey& o) const {
-        return kind == o.kind && T == o.T && real == o.real && loop == o.loop && half == o.half && mtl == o.mtl;
+        return kind == o.kind && T == o.T && real == o.real && loop == o.loop && half == o.half && mtl == o.mtl &&
+               H == o.H && nw == o.nw;
     }
 };
 
@@ -209,7 +220,44 @@ static void pick_matrix_cores(GroupKey& g, int options) {
     }
 }
 
+GroupKey classify_auto(const wh_job_dev& j, int options);
+
+// The LDS bytes of a workgroup of nw waves for an all-ones or repeated-row group.
+size_t group_lds_bytes(const GroupKey& g, int nw) {
+    if (g.kind == KIND_PM1S) return wh_pm1s_lds_bytes(g.T, nw);
+    if (g.kind == KIND_PM1M) return wh_pm1m_lds_bytes(g.T, nw);
+    if (g.kind == KIND_PM1) return wh_pm1_lds_bytes(g.T, g.real != 0, nw);
+    if (g.kind == KIND_REP) return wh_rep_lds_bytes(g.T, g.real != 0, nw, g.H);
+    return 0;
+}
+
+// The kernel class of a request, with the WH_OPT_TILE / WH_OPT_WAVES overrides applied; fails
+// (kind < 0) when an override does not fit the class.
 GroupKey classify(const wh_job_dev& j, int options) {
+    GroupKey g = classify_auto(j, options);
+    const int tile = (options >> WH_OPT_TILE_SHIFT) & 15;
+    const int waves = (options >> WH_OPT_WAVES_SHIFT) & 7;
+    if (tile) {
+        const bool ok = all_ones_kind(g.kind) ? tile >= g.T && tile <= (g.kind == KIND_PM1S ? kPm1sMaxT : kPm1MaxT) &&
+                                                    !(g.kind == KIND_PM1M && tile > 5)
+                        : g.kind == KIND_REP ? tile >= g.T && tile <= kRepMaxT : false;
+        if (!ok) {
+            g.kind = -1;
+            return g;
+        }
+        g.T = tile;
+    }
+    if (waves) {
+        if (!(all_ones_kind(g.kind) || g.kind == KIND_REP) || waves > 4 || group_lds_bytes(g, waves) > 160 * 1024) {
+            g.kind = -1;
+            return g;
+        }
+        g.nw = waves;
+    }
+    return g;
+}
+
+GroupKey classify_auto(const wh_job_dev& j, int options) {
     GroupKey g{KIND_GENERIC, 0, 0, 0, 0, 0};
     if (j.flags & WH_F_MTL) {  // the montrealer runs on the all-ones kernels only
         g.kind = KIND_PM1;
@@ -233,8 +281,41 @@ GroupKey classify(const wh_job_dev& j, int options) {
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
         g.half = (options & WH_OPT_FULL) ? 0 : 1;
         pick_matrix_cores(g, options);
+        return g;
+    }
+    if (all_one && glynn && !odd && T <= kPm1sMaxT && j.E >= 2) {
+        // n in [50, 64]: the symmetric matrix-core kernel only (one wave per CU, the lanes'
+        // accumulators partly in scratch), real inputs as complex ones with zero imaginary parts
+        g.kind = KIND_PM1S;
+        g.T = T;
+        g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
+        g.half = 1;
+        return g;
+    }
+    const int h = j.N / 2;
+    if (!odd && T <= kRepMaxT && h <= kRepMaxH && j.steps < ((int64_t)1 << 31) &&
+        !(j.flags & WH_F_CHARPOLY) && !(options & WH_OPT_FULL)) {
+        g.kind = KIND_REP;
+        g.T = T;
+        g.H = h <= 8 ? 8 : 16;
+        g.real = (j.flags & WH_F_REAL) ? 1 : 0;
+        g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
+        g.half = 1;
     }
     return g;
+}
+
+// Waves per workgroup for a REP group: the count that maximises resident waves per CU under the
+// LDS budget, at most 4 (the kernel's launch bound).
+int rep_waves(int T, bool real, int H) {
+    const size_t lds_cap = 160 * 1024;
+    int best = 1, best_w = 0;
+    for (int nw = 1; nw <= 4; ++nw) {
+        int per_cu = (int)(lds_cap / wh_rep_lds_bytes(T, real, nw, H)) * nw;
+        if (per_cu > 32) per_cu = 32;
+        if (per_cu > best_w) { best_w = per_cu; best = nw; }
+    }
+    return best;
 }
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
@@ -255,12 +336,13 @@ int pm1_waves(int T, bool real, int kind) {
 }
 
 // Workgroups per request the chunk size aims at, per kernel class (wh_tune_set; 0 = the
-// defaults below): [0][T][loop] all-ones kernels by tile side class, [1][0][loop] general kernel.
-std::atomic<int64_t> g_tune_blocks[2][kPm1MaxT + 1][2];
+// defaults below): [0][T][loop] all-ones kernels by tile side class, [1][0][loop] general kernel,
+// [2][T][loop] repeated-row kernel (T = 1, 2).
+std::atomic<int64_t> g_tune_blocks[3][kPm1MaxT + 1][2];
 
 int64_t tuned_blocks(const GroupKey& g) {
-    const int kind = all_ones_kind(g.kind) ? 0 : 1;
-    const int T = kind == 0 ? std::min(std::max(g.T, 0), kPm1MaxT) : 0;
+    const int kind = all_ones_kind(g.kind) ? 0 : g.kind == KIND_REP ? 2 : 1;
+    const int T = kind == 1 ? 0 : std::min(std::max(g.T, 0), kPm1MaxT);
     return g_tune_blocks[kind][T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
 }
 
@@ -269,6 +351,13 @@ int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
     if (all_ones_kind(g.kind)) {
         int64_t c = j.steps / (tb > 0 ? tb : 1024);
         if (c < 4 * nw) c = 4 * nw;
+        if (c > 4096) c = 4096;
+        c = (c + nw - 1) / nw * nw;
+        return c;
+    }
+    if (g.kind == KIND_REP) {  // each wave takes every nw-th term of its workgroup's chunk
+        int64_t c = j.steps / (tb > 0 ? tb : 64);
+        if (c < 8 * nw) c = 8 * nw;
         if (c > 4096) c = 4096;
         c = (c + nw - 1) / nw * nw;
         return c;
@@ -291,9 +380,9 @@ double term_flops(const wh_job_dev& j, const GroupKey& g) {
     int m = 2 * j.E;
     int np = std::min(h + 1, m) - 2;
     if (g.kind == KIND_GENERIC && !(j.flags & WH_F_CHARPOLY)) np = h - 1;  // every power formed
-    if (all_ones_kind(g.kind) && g.half) np = (h + 1) / 2 - 1;
+    if ((all_ones_kind(g.kind) || g.kind == KIND_REP) && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
-    double c = (g.kind == KIND_PM1 && g.real) ? 2.0 : 8.0;
+    double c = ((g.kind == KIND_PM1 || g.kind == KIND_REP) && g.real) ? 2.0 : 8.0;
     if (g.kind == KIND_PM1S) return c * (m * (m + 1.0) / 2.0) * m * np;
     return c * m * (double)m * m * np;
 }
@@ -324,7 +413,11 @@ struct Layout {
     double flops = 0.0;
 };
 
-int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& L) {
+// chunk_hint > 0 fixes the terms per chunk of every request; else chunk_per_req (when given) holds
+// each request's own chunk size, and otherwise the tuned default applies.  Chunk counts and the
+// batch's total stay within kMaxBlocks (a grid's x dimension), or the layout fails.
+int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& L,
+                 const int64_t* chunk_per_req = nullptr) {
     std::vector<wh_job_dev> tmp(n);
     std::vector<GroupKey> keys(n);
     for (int64_t i = 0; i < n; ++i) {
@@ -332,6 +425,8 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
         if (rc) return rc;
         tmp[i].out_index = i;
         keys[i] = classify(tmp[i], reqs[i].options);
+        if (keys[i].kind < 0)
+            return fail(WH_EINVAL, "request %lld: tile side or waves option does not fit its kernel", (long long)i);
         if (keys[i].kind == KIND_GENERIC && 2 * reqs[i].E > kGenericMaxSide)
             return fail(WH_EUNSUPPORTED, "matrix side %d exceeds the generic kernel limit %d",
                         2 * reqs[i].E, kGenericMaxSide);
@@ -350,14 +445,22 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
             gr.key = g;
             gr.job_begin = k;
             gr.block_begin = base;
-            gr.nw = all_ones_kind(g.kind) ? pm1_waves(g.T, g.real != 0, g.kind) : 1;
+            gr.nw = g.nw                  ? g.nw
+                    : all_ones_kind(g.kind) ? pm1_waves(g.T, g.real != 0, g.kind)
+                    : g.kind == KIND_REP  ? rep_waves(g.T, g.real != 0, g.H)
+                                          : 1;
             L.groups.push_back(gr);
         }
         Group& gr = L.groups.back();
         wh_job_dev j = tmp[i];
-        int64_t c = chunk_hint > 0 ? chunk_hint : chunk_for(j, g, gr.nw);
+        int64_t c = chunk_hint > 0 ? chunk_hint : chunk_per_req ? chunk_per_req[i] : chunk_for(j, g, gr.nw);
+        if (c < 1) return fail(WH_EINVAL, "chunk size %lld", (long long)c);
+        const int64_t nch = j.steps / c + (j.steps % c != 0);
+        if (nch > kMaxBlocks || base + nch > kMaxBlocks)
+            return fail(WH_EUNSUPPORTED, "%lld term chunks exceed the launch limit %lld (terms per chunk %lld)",
+                        (long long)(base + nch), (long long)kMaxBlocks, (long long)c);
         j.chunk_terms = c;
-        j.nchunks = (int32_t)((j.steps + c - 1) / c);
+        j.nchunks = (int32_t)nch;
         j.chunk_base = base;
         base += j.nchunks;
         gr.job_end = k + 1;
@@ -374,10 +477,39 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
     return WH_OK;
 }
 
+// Batches run their symmetric-kernel groups of tile sides 4..6 (n in [26, 48]: the classes whose
+// own kernels also run one wave per SIMD) as one persistent launch when `counters` (two device
+// words of the caller's arena) is given and the batch has two such groups or more
+// (wh_tune_persistent: 0 = never, 1 = by that rule, 2 = whenever there is one such group);
+// the other groups have their own launches first.  Without a persistent launch every group has
+// its own launch, then wh_fold.
+std::atomic<int> g_persistent{1};
+std::atomic<int> g_device_cus[kMaxDevices];
+
+int device_cus(int dev) {
+    int cus = g_device_cus[dev].load(std::memory_order_relaxed);
+    if (cus > 0) return cus;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    g_device_cus[dev].store(cus, std::memory_order_relaxed);
+    return cus;
+}
+
+bool mixed_group(const GroupKey& k) { return k.kind == KIND_PM1S && k.T >= 4 && k.T <= 6; }
+
 // Enqueues the sieve kernels of every group and the fold.
 hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* pool, double4* partials,
-                      void* out, hipStream_t s) {
+                      void* out, hipStream_t s, unsigned* counters = nullptr) {
+    std::vector<wh_mixed_group> mixed;
+    const int mode = g_persistent.load(std::memory_order_relaxed);
+    if (counters && mode > 0) {
+        for (const Group& g : L.groups)
+            if (mixed_group(g.key))
+                mixed.push_back(wh_mixed_group{g.block_begin, g.block_end, (int32_t)g.job_begin, (int32_t)g.job_end,
+                                               g.key.T, g.key.loop, g.nw, 0});
+        if (mode == 1 && mixed.size() < 2) mixed.clear();
+    }
     for (const Group& g : L.groups) {
+        if (!mixed.empty() && mixed_group(g.key)) continue;
         const wh_job_dev* gj = d_jobs + g.job_begin;
         int nj = (int)(g.job_end - g.job_begin);
         int64_t nb = g.block_end - g.block_begin;
@@ -391,11 +523,25 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
             e = wh_launch_pm1(g.key.T, g.key.half != 0, g.key.real != 0, g.key.loop != 0, gj, nj, pool,
                               partials, nb,
                               g.block_begin, g.nw, s);
+        else if (g.key.kind == KIND_REP)
+            e = wh_launch_rep(g.key.T, g.key.H, g.key.real != 0, g.key.loop != 0, gj, nj, pool, partials, nb,
+                              g.block_begin, g.nw, s);
         else
             e = wh_launch_generic(gj, nj, pool, partials, nb, g.block_begin, g.MM, g.NQ, s);
         if (e != hipSuccess) return e;
     }
-    return wh_launch_fold(d_jobs, (int)L.jobs.size(), partials, out, s);
+    if (mixed.empty()) return wh_launch_fold(d_jobs, (int)L.jobs.size(), partials, out, s);
+    // largest tile class first: the long chunks start early and the short ones fill the tail
+    std::stable_sort(mixed.begin(), mixed.end(),
+                     [](const wh_mixed_group& a, const wh_mixed_group& b) { return a.T > b.T; });
+    int64_t items = 0;
+    for (const wh_mixed_group& m : mixed) items += m.block_end - m.block_begin;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const int nwg = (int)std::min<int64_t>(items, device_cus(dev < 0 || dev >= kMaxDevices ? 0 : dev));
+    return wh_launch_pm1s_persistent(mixed.data(), (int)mixed.size(), d_jobs, (int)L.jobs.size(), pool, partials,
+                                     counters, out, nwg, s);
 }
 
 }  // namespace
@@ -410,6 +556,7 @@ struct wh_plan {
     Layout layout;
     wh_job_dev* d_jobs = nullptr;
     double4* d_partials = nullptr;
+    unsigned* d_counters = nullptr;  // the persistent launch's two work counters
 };
 
 struct wh_graph {
@@ -540,8 +687,8 @@ int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_pl
     if (rc) { delete p; return rc; }
     const Layout& L = p->layout;
     if (hipMalloc((void**)&p->d_jobs, L.jobs.size() * sizeof(wh_job_dev)) != hipSuccess ||
-        hipMalloc((void**)&p->d_partials, std::max<int64_t>(L.blocks, 1) * sizeof(double4)) !=
-            hipSuccess) {
+        hipMalloc((void**)&p->d_partials, std::max<int64_t>(L.blocks, 1) * sizeof(double4)) != hipSuccess ||
+        hipMalloc((void**)&p->d_counters, 256) != hipSuccess) {
         wh_plan_destroy(p);
         return fail(WH_ENOMEM, "plan allocation failed");
     }
@@ -557,7 +704,7 @@ int wh_plan_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_pl
 int wh_plan_run(wh_plan* p, const double* dpool, double* dout, void* stream) {
     if (!p || !dpool || !dout) return fail(WH_EINVAL, "null plan or buffer");
     hipStream_t s = (hipStream_t)stream;  // NULL: the device's default stream
-    HIP_TRY(run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s));
+    HIP_TRY(run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s, p->d_counters));
     return WH_OK;
 }
 
@@ -593,6 +740,7 @@ int wh_plan_destroy(wh_plan* p) {
     if (!p) return WH_OK;
     if (p->d_jobs) (void)hipFree(p->d_jobs);
     if (p->d_partials) (void)hipFree(p->d_partials);
+    if (p->d_counters) (void)hipFree(p->d_counters);
     delete p;
     return WH_OK;
 }
@@ -608,7 +756,7 @@ int wh_plan_graph_create(wh_plan* p, const double* dpool, double* dout, void* st
         delete g;
         return fail(WH_EDEVICE, "stream capture failed to start: %s", hipGetErrorString(e));
     }
-    hipError_t er = run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s);
+    hipError_t er = run_layout(p->layout, p->d_jobs, dpool, p->d_partials, dout, s, p->d_counters);
     hipError_t ee = hipStreamEndCapture(s, &g->graph);
     if (er != hipSuccess || ee != hipSuccess) {
         wh_graph_destroy(g);
@@ -639,17 +787,23 @@ int wh_graph_destroy(wh_graph* g) {
 }
 
 int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request) {
-    if (kind < 0 || kind > 1 || loop < 0 || loop > 1 || blocks_per_request < 0 ||
-        (kind == 0 && (T < 1 || T > kPm1MaxT)) || (kind == 1 && T != 0))
+    if (kind < 0 || kind > 2 || loop < 0 || loop > 1 || blocks_per_request < 0 ||
+        (kind == 0 && (T < 1 || T > kPm1MaxT)) || (kind == 1 && T != 0) || (kind == 2 && (T < 1 || T > kRepMaxT)))
         return fail(WH_EINVAL, "tune entry out of range (kind %d, T %d, loop %d, blocks %lld)", kind, T, loop,
                     (long long)blocks_per_request);
     g_tune_blocks[kind][T][loop].store(blocks_per_request, std::memory_order_relaxed);
     return WH_OK;
 }
 
+int wh_tune_persistent(int mode) {
+    if (mode > 2) return fail(WH_EINVAL, "persistent mode %d out of range", mode);
+    if (mode < 0) return g_persistent.load(std::memory_order_relaxed);
+    return g_persistent.exchange(mode, std::memory_order_relaxed);
+}
+
 int64_t wh_tune_get(int kind, int T, int loop) {
-    if (kind < 0 || kind > 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
-        (kind == 1 && T != 0))
+    if (kind < 0 || kind > 2 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
+        (kind == 1 && T != 0) || (kind == 2 && (T < 1 || T > kRepMaxT)))
         return -1;
     return g_tune_blocks[kind][T][loop].load(std::memory_order_relaxed);
 }
@@ -816,6 +970,10 @@ static int lhb_call(const double* A, const double* D, int64_t nD, int64_t twoE, 
     int64_t steps = (batch_max + 1) * (odd_variant ? 2 : 1);
     for (int64_t i = 0; i < Ef; ++i) steps *= fixed_edge_reps[i] + 1;
     int64_t chunk = std::min<int64_t>(std::max<int64_t>(steps / 1024, 1), 256);
+    const int64_t nch = steps / chunk + (steps % chunk != 0);
+    if (nch > kMaxBlocks / nD)
+        return fail(WH_EUNSUPPORTED, "%lld term chunks (%lld terms) exceed the launch limit", (long long)nch,
+                    (long long)steps);
     j.E = (int32_t)E;
     j.L = (int32_t)nL;
     j.nD = (int32_t)nD;
@@ -824,7 +982,7 @@ static int lhb_call(const double* A, const double* D, int64_t nD, int64_t twoE, 
     j.flags = (odd_variant ? WH_LHB_ODD : 0) | (glynn ? WH_LHB_GLYNN : 0);
     j.steps = steps;
     j.chunk_terms = chunk;
-    j.nchunks = (int32_t)((steps + chunk - 1) / chunk);
+    j.nchunks = (int32_t)nch;
     // few chunks (small term counts): one workgroup per chunk and diagonal, so the fanout rows
     // run in parallel (each repeats the chunk's shared per-term traces)
     j.dsplit = (nD > 1 && j.nchunks < 1024) ? (int32_t)nD : 1;
@@ -1005,6 +1163,7 @@ struct SrvItem {
     wh_request req;
     std::vector<double> payload;  // 2 * elems doubles
     int64_t elems = 0, slots = 0, footprint = 0;
+    int64_t chunk = 0;  // terms per chunk, fixed at submission (admission counted its slots)
     double t_submit = 0.0, t_deadline = -1.0;
 };
 
@@ -1025,6 +1184,7 @@ struct SrvSlot {
     double4* d_partials = nullptr;
     double* h_out = nullptr;
     double* d_out = nullptr;
+    unsigned* d_counters = nullptr;  // the persistent launch's two work counters
     std::vector<uint64_t> tags;
     std::vector<double> t_submit;
     double t_launch = 0.0;
@@ -1068,6 +1228,7 @@ void srv_free(wh_server* s) {
         if (sl.d_jobs) (void)hipFree(sl.d_jobs);
         if (sl.d_partials) (void)hipFree(sl.d_partials);
         if (sl.d_out) (void)hipFree(sl.d_out);
+        if (sl.d_counters) (void)hipFree(sl.d_counters);
     }
     s->slots.clear();
 }
@@ -1077,8 +1238,10 @@ void srv_free(wh_server* s) {
 void srv_launch(wh_server* s, SrvSlot& sl, std::vector<SrvItem>& items, int64_t fp) {
     const int64_t n = (int64_t)items.size();
     std::vector<wh_request> reqs(n);
+    std::vector<int64_t> chunks(n);
     int64_t off = 0;
     for (int64_t k = 0; k < n; ++k) {
+        chunks[k] = items[k].chunk;
         wh_request r = items[k].req;
         r.a_off += off;
         if (r.d_off >= 0) r.d_off += off;
@@ -1088,7 +1251,8 @@ void srv_launch(wh_server* s, SrvSlot& sl, std::vector<SrvItem>& items, int64_t 
         off += items[k].elems;
     }
     Layout L;
-    int rc = build_layout(reqs.data(), n, s->chunk_terms, L);
+    // each request keeps the chunking it was admitted with, whatever the tuning table says now
+    int rc = build_layout(reqs.data(), n, 0, L, chunks.data());
     if (rc == WH_OK && (L.blocks > s->partial_slots || (int64_t)L.jobs.size() > s->max_requests))
         rc = fail(WH_EINVAL, "batch exceeds the slot arena");
     hipError_t e = hipSuccess;
@@ -1098,7 +1262,8 @@ void srv_launch(wh_server* s, SrvSlot& sl, std::vector<SrvItem>& items, int64_t 
         if (e == hipSuccess)
             e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, L.jobs.size() * sizeof(wh_job_dev), hipMemcpyHostToDevice,
                                sl.stream);
-        if (e == hipSuccess) e = run_layout(L, sl.d_jobs, sl.d_pool, sl.d_partials, sl.d_out, sl.stream);
+        if (e == hipSuccess)
+            e = run_layout(L, sl.d_jobs, sl.d_pool, sl.d_partials, sl.d_out, sl.stream, sl.d_counters);
         if (e == hipSuccess)
             e = hipMemcpyAsync(sl.h_out, sl.d_out, (size_t)n * 16, hipMemcpyDeviceToHost, sl.stream);
         if (e == hipSuccess) e = hipEventRecord(sl.event, sl.stream);
@@ -1248,6 +1413,7 @@ int wh_server_create(int device, int nslots, int64_t pool_elems, int64_t partial
         if (e == hipSuccess) e = hipMalloc((void**)&sl.d_partials, (size_t)partial_slots * sizeof(double4));
         if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_out, (size_t)max_requests * 16, hipHostMallocDefault);
         if (e == hipSuccess) e = hipMalloc((void**)&sl.d_out, (size_t)max_requests * 16);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_counters, 256);
     }
     if (e != hipSuccess) {
         srv_free(s);
@@ -1290,6 +1456,7 @@ int srv_enqueue(wh_server* s, const wh_request& req, std::vector<double>&& paylo
     it.payload = std::move(payload);
     it.elems = elems;
     it.slots = L.blocks;
+    it.chunk = L.jobs[0].chunk_terms;
     it.footprint = fp;
     it.t_submit = steady_now();
     it.t_deadline = deadline_s >= 0.0 ? it.t_submit + deadline_s : -1.0;

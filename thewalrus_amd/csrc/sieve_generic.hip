@@ -235,31 +235,12 @@ wh_sieve_generic(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __r
     if (lane == acc_lane) partials[b] = make_double4(acc.sr, acc.cr, acc.si, acc.ci);
 }
 
-// Folds each job's chunk partials in chunk order with compensation and applies the sieve's
-// power-of-two scale (_hafnian.py:464-465, :571-575).  One thread per job.
+// Folds each job's chunk partials in chunk order (wh_fold_job).  One thread per job.
 extern "C" __global__ void wh_fold(const wh_job_dev* __restrict__ jobs, int njobs,
                                    const double4* __restrict__ partials, double2* __restrict__ out) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= njobs) return;
-    const wh_job_dev& J = jobs[q];
-    csum acc;
-    acc.zero();
-    for (int c = 0; c < J.nchunks; ++c) {
-        double4 p = partials[J.chunk_base + c];
-        acc.add_pair(p.x, p.y, p.z, p.w);
-    }
-    double re = acc.sr + acc.cr, im = acc.si + acc.ci;
-    if ((J.flags & WH_F_MTL) && (J.E & 1) == 0) {  // (-1)^(n+1), _montrealer.py:57, :101
-        re = -re;
-        im = -im;
-    }
-    if (J.flags & WH_F_GLYNN) {
-        int ex = (J.flags & WH_F_ODD) ? J.N / 2 : J.N / 2 - 1;
-        double sc = ldexp(1.0, -ex);
-        re *= sc;
-        im *= sc;
-    }
-    out[J.out_index] = make_double2(re, im);
+    wh_fold_job(jobs[q], partials, out);
 }
 
 // ---- host-side launchers (kernels are launched from the translation unit that defines them)

@@ -499,6 +499,249 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
     }
 }
 
+// ---- Repeated rows: any edge multiplicities, even N, Glynn or inclusion-exclusion, on the same
+// register-tiled products.  Term t has mixed-radix digits k_i (digit 0 most significant, base
+// e_i + 1, _hafnian.py:162-180) and delta_i = 2 k_i - e_i (Glynn) or k_i.  The reference keeps
+// the rows and columns of Ax X with delta_i != 0 and scales the columns by delta (AX_S,
+// :288-312); here every term works on the padded B = M diag(d), M = Ax X, d_c = delta_(c mod E).
+// A column with delta = 0 is zero, so under a permutation B is block triangular with AX_S as
+// its kept block, and tr(B^k) = tr(AX_S^k); the loop weights XD_d = (X D) diag(d) vanish on the
+// same indices, so XD_d B^(k-1) D = XD_S AX_S^(k-1) D_S.  The powers run to B^ceil(h/2) with
+// h = N/2 as in the all-ones kernel (direct powers where the reference extends the traces by
+// the characteristic polynomial, charpoly.py:321-326), and the term is
+// (-1)^(h - sum k) prod C(e_i, k_i) (1/2 for Glynn when delta_0 = 0) [eta^h] exp(sum a_i eta^i)
+// (_hafnian.py:444-460, :546-569).  Every job of a launch has 2E <= 8T, h <= H and fewer than
+// 2^31 terms (the host checks).
+__host__ __device__ constexpr int rep_nb() { return 8; }
+__host__ __device__ constexpr int rep_ptn(int H) { return H + 4; }
+
+template <int T, bool REAL, bool LOOP, int H>
+__device__ __forceinline__ void rep_body(const wh_job_dev& J, const cplx* __restrict__ pool,
+                                         double4* __restrict__ partials, int64_t b, unsigned char* smem) {
+    using E = typename Elem<REAL>::T;
+    constexpr int NP = 8 * T;
+    constexpr int S = Lds<T>::S;
+    constexpr int PTN = rep_ptn(H);
+    constexpr int NB = rep_nb();
+    const int nw = blockDim.x / WH_WAVE;
+    const int wave = threadIdx.x / WH_WAVE;
+    const int lane = threadIdx.x % WH_WAVE;
+    const int ti = lane >> 3, tj = lane & 7;
+    const int ne = J.E;
+    const int n = 2 * ne;
+    const int h = J.N / 2;
+    const bool glynn = (J.flags & WH_F_GLYNN) != 0;
+
+    // ---- LDS carve: M, one power per wave, XD0 / D, per-wave term slots, per-wave results,
+    // the binomial table C(e_i, k) (row i, k = 0..H)
+    E* Ms = reinterpret_cast<E*>(smem);
+    E* Pw = Ms + NP * S + wave * NP * S;
+    E* vecs = Ms + (1 + nw) * NP * S;
+    E* XD0 = vecs;
+    E* Dv = vecs + NP;
+    cplx* slots = reinterpret_cast<cplx*>(vecs + 2 * NP) + wave * (NB * 2 * PTN);
+    double4* wres = reinterpret_cast<double4*>(reinterpret_cast<cplx*>(vecs + 2 * NP) + nw * (NB * 2 * PTN));
+    double* btab = reinterpret_cast<double*>(wres + nw);
+
+    const cplx* A = pool + J.a_off;
+    for (int e = threadIdx.x; e < NP * NP; e += blockDim.x) {
+        int r = e / NP, c = e - r * NP;
+        E v;
+        ezero(v);
+        if (r < n && c < n) {
+            int src = c < ne ? c + ne : c - ne;
+            v = eload(A, (int64_t)r * n + src, (E*)nullptr);
+        }
+        Ms[r * S + c] = v;
+    }
+    if (LOOP) {
+        const cplx* Dg = pool + J.d_off;
+        for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+            E xd, dd;
+            ezero(xd);
+            ezero(dd);
+            if (e < n) {
+                int src = e < ne ? e + ne : e - ne;  // (X D)[e]
+                xd = eload(Dg, src, (E*)nullptr);
+                dd = eload(Dg, e, (E*)nullptr);
+            }
+            XD0[e] = xd;
+            Dv[e] = dd;
+        }
+    }
+    for (int e = threadIdx.x; e < ne * (H + 1); e += blockDim.x) {
+        int i = e / (H + 1), k = e - i * (H + 1);
+        btab[e] = binom_d(J.e[i], k);
+    }
+    __syncthreads();
+
+    const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
+    int64_t t1 = t0 + J.chunk_terms;
+    if (t1 > J.steps) t1 = J.steps;
+    const int I0 = ti * T, J0 = tj * T;
+    // lane i < ne owns digit i: its base and the weight of the digits after it
+    const int my_e = lane < ne ? J.e[lane] : 0;
+    uint32_t my_w = 1;
+    for (int l = lane + 1; l < ne; ++l) my_w *= (uint32_t)(J.e[l] + 1);
+    // the edge index of column (and row) c, for c < n: c mod ne
+    int srcJ[T], srcI[T];
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+        int col = J0 + c, row = I0 + c;
+        srcJ[c] = col < ne ? col : col < n ? col - ne : 0;
+        srcI[c] = row < ne ? row : row < n ? row - ne : 0;
+    }
+    const int src_l = lane < ne ? lane : lane < n ? lane - ne : 0;
+
+    csum acc_sum;
+    acc_sum.zero();
+
+    int slot = 0;
+    for (int64_t t = t0 + wave; t < t1; t += nw) {
+        const uint32_t tu = (uint32_t)t;
+        // _hafnian.py:444-459: digits, sign, binomial product, delta
+        const int k = lane < ne ? (int)((tu / my_w) % (uint32_t)(my_e + 1)) : 0;
+        const int delta = lane < ne ? (glynn ? 2 * k - my_e : k) : 0;
+        int ksum = k;
+        double bin = lane < ne ? btab[lane * (H + 1) + k] : 1.0;
+#pragma unroll
+        for (int msk = 1; msk < 8; msk <<= 1) {  // ne <= 8: lanes 0..7 hold every digit
+            ksum += __shfl_xor(ksum, msk, 8);
+            bin *= __shfl_xor(bin, msk, 8);  // exact: a product of integers below 2^53
+        }
+        ksum = __shfl(ksum, 0, WH_WAVE);
+        bin = __shfl(bin, 0, WH_WAVE);
+        const int delta0 = __shfl(delta, 0, WH_WAVE);
+        double prefac = (((h - ksum) & 1) ? -1.0 : 1.0) * bin;
+        if (glynn && delta0 == 0) prefac *= 0.5;
+        double sgnJ[T], sgnI[T];
+#pragma unroll
+        for (int c = 0; c < T; ++c) {
+            const double dj = (double)__shfl(delta, srcJ[c], WH_WAVE);
+            const double di = (double)__shfl(delta, srcI[c], WH_WAVE);
+            sgnJ[c] = J0 + c < n ? dj : 0.0;
+            sgnI[c] = I0 + c < n ? di : 0.0;
+        }
+        const double dl = (double)__shfl(delta, src_l, WH_WAVE);
+
+        cplx* pt = slots + slot * 2 * PTN;  // pt[0] = (prefactor, 0); p[1..h]
+        cplx* lv = pt + PTN;                // loop scalars v[1..h]
+        E acc[T][T];
+#pragma unroll
+        for (int r = 0; r < T; ++r)
+#pragma unroll
+            for (int c = 0; c < T; ++c) acc[r][c] = escale(Ms[(I0 + r) * S + J0 + c], sgnJ[c]);
+        auto load_xd = [&](E (&xdI)[T]) {
+#pragma unroll
+            for (int r = 0; r < T; ++r) xdI[r] = escale(XD0[I0 + r], sgnI[r]);
+        };
+        auto load_d = [&](E (&dJ)[T]) {
+#pragma unroll
+            for (int c = 0; c < T; ++c) dJ[c] = Dv[J0 + c];
+        };
+        // powers up to B^A, A = ceil(h/2): tr(B^(2a)) = <B^a, (B^a)^T>, tr(B^(2a+1)) =
+        // <B^(a+1), (B^a)^T>; loop scalars v_(2a) = y_a . z_(a-1), v_(2a+1) = y_a . z_a with
+        // y_a = XD_d B^a and z_a = B^a D (as in the all-ones kernel)
+        const int Ah = (h + 1) / 2;
+        E zprev[T];
+        if (LOOP) {
+#pragma unroll
+            for (int r = 0; r < T; ++r) zprev[r] = Dv[I0 + r];
+        }
+        tile_store<T>(Pw, acc, I0, J0);
+        for (int a = 1; a <= Ah; ++a) {
+            E qo, qe;
+            if (a == 1) {
+                qo = tile_trace<T>(acc, ti == tj);
+                qe = tile_tdot<T>(Pw, acc, I0, J0);
+            } else {
+                tile_product<T>(Pw, Ms, acc, I0, J0, n, sgnJ);
+                qo = tile_tdot<T>(Pw, acc, I0, J0);
+                tile_store<T>(Pw, acc, I0, J0);
+                qe = tile_tdot<T>(Pw, acc, I0, J0);
+            }
+            cplx qoc = wave_csum(etoc(qo));
+            cplx qec = wave_csum(etoc(qe));
+            const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+            if (lane == 0) {
+                pt[ko] = qoc;
+                pt[ke] = qec;  // ke <= h + 1 < PTN
+            }
+            if (LOOP) {
+                E y[T], z[T], xdI[T], dJ[T];
+                load_xd(xdI);
+                load_d(dJ);
+                tile_yvec<T>(acc, xdI, y);
+                tile_zvec<T>(acc, dJ, z);
+                E de, dodd;
+                ezero(de);
+                ezero(dodd);
+                if (ti == tj) {
+#pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        emac(de, y[c], zprev[c]);
+                        emac(dodd, y[c], z[c]);
+                    }
+                }
+                cplx vec_ = wave_csum(etoc(de));
+                cplx voc_ = wave_csum(etoc(dodd));
+                if (lane == 0) {
+                    if (2 * a <= h) lv[2 * a] = vec_;
+                    if (2 * a + 1 <= h) lv[2 * a + 1] = voc_;
+                }
+#pragma unroll
+                for (int r = 0; r < T; ++r) zprev[r] = z[r];
+            }
+        }
+        if (LOOP) {
+            // v_1 = XD_d . D (n <= 64: lane e holds index e)
+            E part;
+            ezero(part);
+            if (lane < n) emac(part, escale(XD0[lane], dl), Dv[lane]);
+            cplx v1 = wave_csum(etoc(part));
+            if (lane == 0) lv[1] = v1;
+        }
+        if (lane == 0) pt[0] = cmk(prefac, 0.0);
+        wave_lds_sync();  // pt and lv as lane 0 stored them
+
+        // the coefficients of up to NB terms together, lane j for the batch's j-th term
+        ++slot;
+        if (slot == NB || t + nw >= t1) {
+            wave_lds_sync();
+            if (lane < slot) {
+                const cplx* ptj = slots + lane * 2 * PTN;
+                const E top = coeff_top_e<E, H, LOOP>(ptj, ptj + PTN, h);
+                acc_sum.add(cmulr(etoc(top), ptj[0].re));
+            }
+            wave_lds_sync();  // the slots are free again
+            slot = 0;
+        }
+    }
+
+    double4* lsum = reinterpret_cast<double4*>(slots);
+    if (lane < NB) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    wave_lds_sync();
+    if (lane == 0) {
+        csum s;
+        s.zero();
+        for (int j = 0; j < NB; ++j) {
+            double4 pq = lsum[j];
+            s.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        wres[wave] = make_double4(s.sr, s.cr, s.si, s.ci);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        csum s;
+        s.zero();
+        for (int w = 0; w < nw; ++w) {
+            double4 p = wres[w];
+            s.add_pair(p.x, p.y, p.z, p.w);
+        }
+        partials[b] = make_double4(s.sr, s.cr, s.si, s.ci);
+    }
+}
+
 }  // namespace
 
 template <int T, bool REAL, bool LOOP, bool HALF>
@@ -566,4 +809,58 @@ hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_de
         case 6: return launch_h<6>(half, real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+
+template <int T, bool REAL, bool LOOP, int H>
+__global__ void __launch_bounds__(256)
+wh_sieve_rep(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
+             double4* __restrict__ partials, int64_t block_offset) {
+    extern __shared__ __align__(16) unsigned char wh_smem[];
+    const int64_t b = blockIdx.x + block_offset;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
+    }
+    rep_body<T, REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
+}
+
+size_t wh_rep_lds_bytes(int T, bool real, int nw, int H) {
+    size_t esz = real ? sizeof(double) : sizeof(cplx);
+    size_t NP = 8 * (size_t)T, S = NP + WH_PM1_PAD;
+    return (1 + nw) * NP * S * esz + 2 * NP * esz + nw * (size_t)(rep_nb() * 2 * rep_ptn(H)) * sizeof(cplx) +
+           nw * sizeof(double4) + (size_t)NP / 2 * (H + 1) * sizeof(double) + 32;
+}
+
+template <int T, bool REAL, bool LOOP, int H>
+static hipError_t launch_rep_t(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                               int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+    size_t lds = wh_rep_lds_bytes(T, REAL, nw, H);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_rep<T, REAL, LOOP, H>, lds, lim);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((wh_sieve_rep<T, REAL, LOOP, H>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds,
+                       stream, jobs, njobs, (const cplx*)pool, (double4*)partials, block_offset);
+    return hipGetLastError();
+}
+
+template <int T, int H>
+static hipError_t launch_rep_rl(bool real, bool loop, const wh_job_dev* jobs, int njobs, const void* pool,
+                                void* partials, int64_t nblocks, int64_t off, int nw, hipStream_t s) {
+    if (real) {
+        if (loop) return launch_rep_t<T, true, true, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+        return launch_rep_t<T, true, false, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    }
+    if (loop) return launch_rep_t<T, false, true, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    return launch_rep_t<T, false, false, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+}
+
+hipError_t wh_launch_rep(int T, int H, bool real, bool loop, const wh_job_dev* jobs, int njobs, const void* pool,
+                         void* partials, int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+    if (T == 1 && H == 8) return launch_rep_rl<1, 8>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    if (T == 1 && H == 16) return launch_rep_rl<1, 16>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    if (T == 2 && H == 8) return launch_rep_rl<2, 8>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    if (T == 2 && H == 16) return launch_rep_rl<2, 16>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    return hipErrorInvalidValue;
 }

@@ -23,6 +23,8 @@
 // y = lane >> 4; group (p, q) holds R[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
 // rows and columns live in one mask register and flip the sign bit of the values they scale;
 // the product's -Im(a) Im(b) term uses the matrix core's negate modifier on B.
+#include <algorithm>
+
 #include "wh_device.h"
 #include "wh_launch.h"
 
@@ -84,8 +86,8 @@ template <int T> struct GeoS {
     // entry the spare that lanes without a value write
     static constexpr int PTN = 4 * T + 4;
     // terms whose coefficients are formed together, one per lane (LDS budget at 4 waves, T <= 5;
-    // T = 6 runs 3 waves with the LDS nearly full)
-    static constexpr int NB = T == 6 ? 1 : T == 5 ? 7 : 8;
+    // T = 6 runs 3 waves with the LDS nearly full, T = 7 and 8 one wave)
+    static constexpr int NB = T >= 6 ? 1 : T == 5 ? 7 : 8;
 };
 
 struct LaneMapS {
@@ -384,13 +386,16 @@ __device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const doubl
     }
 }
 
+// One chunk (partial slot b) of job J by the first nw waves of the workgroup; any further waves
+// only take part in the staging and the barriers (the persistent form runs every class with the
+// largest class's workgroup).
 template <int T, bool LOOP>
 __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __restrict__ pool,
-                                          double4* __restrict__ partials, int64_t b, unsigned char* smem) {
+                                          double4* __restrict__ partials, int64_t b, unsigned char* smem, int nw) {
     constexpr int NP = GeoS<T>::NP;
     constexpr int S = GeoS<T>::S;
-    const int nw = blockDim.x / WH_WAVE;
     const int wave = threadIdx.x / WH_WAVE;
+    const bool active = wave < nw;
     const int lane = threadIdx.x % WH_WAVE;
     LaneMapS L;
     L.x = lane & 3;
@@ -438,7 +443,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     // the j-th term of the batch (the exponential recurrence is the same work for one term or 64).
     int slot = 0;
     int64_t t_batch = t0 + wave;  // the batch's first term
-    for (int64_t t = t0 + wave; t < t1; t += nw) {
+    for (int64_t t = t0 + wave; active && t < t1; t += nw) {
         const uint64_t tu = (uint64_t)t;
         cplx* pt = slots + slot * 2 * PTN;
         cplx* lv = pt + PTN;
@@ -534,9 +539,9 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     // lanes 0 .. NB-1 hold the partial sums of their terms: fold them in lane order, then the
     // waves of this workgroup in wave order
     double4* lsum = reinterpret_cast<double4*>(slots);
-    if (lane < NB) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    if (active && lane < NB) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
     wave_lds_sync();
-    if (lane == 0) {
+    if (active && lane == 0) {
         csum s;
         s.zero();
         for (int j = 0; j < NB; ++j) {
@@ -570,12 +575,87 @@ wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
     }
-    pm1s_body<T, LOOP>(jobs[lo], pool, partials, b, wh_smem);
+    pm1s_body<T, LOOP>(jobs[lo], pool, partials, b, wh_smem, blockDim.x / WH_WAVE);
 }
+
+// ---- Persistent form for a whole batch: the chunks of the symmetric-kernel groups with tile
+// sides 4..6 (the per-class launches' blocks, the same chunking and the same per-chunk work,
+// so every partial is bit-identical) are items of one queue, largest class first; each
+// workgroup of the grid (as many as stay resident) takes items from an atomic counter until the
+// queue is empty.  The last workgroup to finish (a second counter) folds every job of the batch
+// in chunk order, as wh_fold does.  A one-thread kernel clears the two counters first (a kernel,
+// not a memset, so the pair can sit in a captured graph).  Other kernel classes of the batch
+// run before them on the same stream.
+struct MixedItems {
+    int ngroups;
+    int njobs;  // jobs folded at the end (the whole batch)
+    wh_mixed_group g[WH_MIXED_MAX_GROUPS];
+};
+
+template <int T, bool LOOP>
+__device__ __noinline__ void mixed_item(const wh_job_dev* __restrict__ jobs, int lo, int hi, const cplx* __restrict__ pool,
+                                        double4* __restrict__ partials, int64_t b, unsigned char* smem, int nw) {
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
+    }
+    pm1s_body<T, LOOP>(jobs[lo], pool, partials, b, smem, nw);
+}
+
+__global__ void __launch_bounds__(256)
+wh_sieve_pm1s_persistent(const wh_job_dev* __restrict__ jobs, const cplx* __restrict__ pool,
+                         double4* __restrict__ partials, unsigned* __restrict__ counters, double2* __restrict__ out,
+                         MixedItems items) {
+    extern __shared__ __align__(16) unsigned char wh_smem[];
+    __shared__ int s_item;
+    __shared__ int s_last;
+    int total = 0;
+    for (int k = 0; k < items.ngroups; ++k) total += (int)(items.g[k].block_end - items.g[k].block_begin);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&counters[0], 1u);
+        __syncthreads();  // also: the previous item's last LDS reads are done
+        int i = s_item;
+        __syncthreads();
+        if (i >= total) break;
+        int k = 0;
+        while (i >= (int)(items.g[k].block_end - items.g[k].block_begin)) {
+            i -= (int)(items.g[k].block_end - items.g[k].block_begin);
+            ++k;
+        }
+        const wh_mixed_group& G = items.g[k];
+        const int64_t b = G.block_begin + i;
+        const int lo = G.job_begin, hi = G.job_end - 1;
+#define WH_MIXED_CASE(TT)                                                                   \
+    case TT:                                                                                \
+        if (G.loop) mixed_item<TT, true>(jobs, lo, hi, pool, partials, b, wh_smem, G.nw);  \
+        else mixed_item<TT, false>(jobs, lo, hi, pool, partials, b, wh_smem, G.nw);        \
+        break;
+        switch (G.T) {
+            WH_MIXED_CASE(1)
+            WH_MIXED_CASE(2)
+            WH_MIXED_CASE(3)
+            WH_MIXED_CASE(4)
+            WH_MIXED_CASE(5)
+            WH_MIXED_CASE(6)
+            default: break;
+        }
+#undef WH_MIXED_CASE
+    }
+    // the partials this workgroup wrote are visible device-wide before it counts itself done
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&counters[1], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int q = threadIdx.x; q < items.njobs; q += blockDim.x) wh_fold_job(jobs[q], partials, out);
+}
+
+__global__ void wh_clear_counters(unsigned* __restrict__ counters) { counters[threadIdx.x] = 0u; }
 
 size_t wh_pm1s_lds_bytes(int T, int nw) {
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1S_PAD;
-    size_t PTN = 4 * (size_t)T + 4, NB = T == 6 ? 1 : T == 5 ? 7 : 8;
+    size_t PTN = 4 * (size_t)T + 4, NB = T >= 6 ? 1 : T == 5 ? 7 : 8;
     size_t mat = (NP * S * sizeof(cplx) + 31) / 32 * 32;
     size_t small = (NB * 2 * PTN + 2 * NP) * sizeof(cplx);
     return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
@@ -593,6 +673,28 @@ static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, 
     return hipGetLastError();
 }
 
+hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, const wh_job_dev* jobs, int njobs,
+                                     const void* pool, void* partials, unsigned* counters, void* out, int nwg,
+                                     hipStream_t stream) {
+    if (ngroups < 1 || ngroups > WH_MIXED_MAX_GROUPS || nwg < 1) return hipErrorInvalidValue;
+    MixedItems items;
+    items.ngroups = ngroups;
+    items.njobs = njobs;
+    size_t lds = 0;
+    for (int k = 0; k < ngroups; ++k) {
+        items.g[k] = groups[k];
+        if (groups[k].T < 1 || groups[k].T > 6 || groups[k].nw < 1 || groups[k].nw > 4) return hipErrorInvalidValue;
+        lds = std::max(lds, wh_pm1s_lds_bytes(groups[k].T, groups[k].nw));
+    }
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s_persistent, lds, lim);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(wh_clear_counters, dim3(1), dim3(2), 0, stream, counters);
+    hipLaunchKernelGGL(wh_sieve_pm1s_persistent, dim3((unsigned)nwg), dim3(256), lds, stream, jobs,
+                       (const cplx*)pool, (double4*)partials, counters, (double2*)out, items);
+    return hipGetLastError();
+}
+
 hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
                           int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
 #define WH_PM1S_CASE(TT)                                                                                   \
@@ -606,6 +708,8 @@ hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, c
         WH_PM1S_CASE(4)
         WH_PM1S_CASE(5)
         WH_PM1S_CASE(6)
+        WH_PM1S_CASE(7)
+        WH_PM1S_CASE(8)
         default: return hipErrorInvalidValue;
     }
 #undef WH_PM1S_CASE

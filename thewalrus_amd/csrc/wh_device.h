@@ -55,6 +55,16 @@ struct __attribute__((aligned(16))) wh_lhb_job {
     int32_t e[WH_EMAX];
 };
 
+// One kernel-class group of a batch for the persistent symmetric-kernel launch: its partial
+// slots [block_begin, block_end) and jobs [job_begin, job_end) (sorted by chunk_base), tile side
+// T, loop flag and waves per workgroup.
+#define WH_MIXED_MAX_GROUPS 12
+struct wh_mixed_group {
+    int64_t block_begin, block_end;
+    int32_t job_begin, job_end;
+    int32_t T, loop, nw, pad;
+};
+
 #define WH_LHB_ODD 1
 #define WH_LHB_GLYNN 2
 
@@ -190,4 +200,28 @@ __device__ __forceinline__ double binom_d(int n, int k) {
         b *= num / (den / x);
     }
     return (double)b;
+}
+
+// Folds one job's chunk partials in chunk order with compensation and applies the sieve's
+// power-of-two scale (_hafnian.py:464-465, :571-575) and the montrealer's sign.
+__device__ __forceinline__ void wh_fold_job(const wh_job_dev& J, const double4* __restrict__ partials,
+                                            double2* __restrict__ out) {
+    csum acc;
+    acc.zero();
+    for (int c = 0; c < J.nchunks; ++c) {
+        double4 p = partials[J.chunk_base + c];
+        acc.add_pair(p.x, p.y, p.z, p.w);
+    }
+    double re = acc.sr + acc.cr, im = acc.si + acc.ci;
+    if ((J.flags & WH_F_MTL) && (J.E & 1) == 0) {  // (-1)^(n+1), _montrealer.py:57, :101
+        re = -re;
+        im = -im;
+    }
+    if (J.flags & WH_F_GLYNN) {
+        int ex = (J.flags & WH_F_ODD) ? J.N / 2 : J.N / 2 - 1;
+        double sc = ldexp(1.0, -ex);
+        re *= sc;
+        im *= sc;
+    }
+    out[J.out_index] = make_double2(re, im);
 }

@@ -9,7 +9,10 @@ the built-in default by more than 1% (`choose`):
 
 * kind 0: the all-ones kernels (every multiplicity 1, even N), by tile side class T = 1..6
   (n_pad = 8T), plain and loop;
-* kind 1: the general kernel (repeated rows, odd N, inclusion-exclusion), plain and loop.
+* kind 1: the general kernel (odd N, and repeated rows beyond the repeated-row kernel's
+  limits), plain and loop;
+* kind 2: the repeated-row kernel (even N, 2E <= 16, h <= 16), by tile side class T = 1, 2,
+  plain and loop.
 
 The table lives in a JSON file (`tuning.json` next to this module by default, or the path in
 the environment variable ``WH_TUNING``; ``WH_TUNING=none`` keeps the built-in defaults) that the
@@ -27,6 +30,9 @@ from . import _native
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "tuning.json")
 CANDIDATES = (256, 512, 1024, 2048, 4096)
+# per kind, when the caller names no candidates: the repeated-row kernel's requests have hundreds
+# of terms, the general kernel's chunks at most 256 terms
+KIND_CANDIDATES = {0: CANDIDATES, 1: (64, 128, 256, 512, 1024), 2: (4, 8, 16, 32, 64)}
 
 
 def table_path():
@@ -52,7 +58,7 @@ def current(lib=None):
     """The library's table: {(kind, T, loop): blocks_per_request}, 0 = built-in default."""
     L = lib or _native.load()
     out = {}
-    for kind, Ts in ((0, range(1, 7)), (1, (0,))):
+    for kind, Ts in ((0, range(1, 7)), (1, (0,)), (2, (1, 2))):
         for T in Ts:
             for loop in (0, 1):
                 out[(kind, T, loop)] = int(L.wh_tune_get(kind, T, loop))
@@ -85,10 +91,21 @@ def _class_workload(kind, T, loop, rng):
         h = n // 2
         count = int(min(256, max(1, 2 ** 18 // 2 ** (h - 1))))
         return [B.prepare(sym(n), loop=bool(loop)) for _ in range(count)]
-    # general kernel: repeated rows, 8 modes, 32 photons (the repeated-row request shape)
+    if kind == 2:
+        # repeated-row kernel: 32 photons on 4 modes (2E <= 8, T = 1) or 8 modes (2E in 10..16, T = 2)
+        modes = 4 if T == 1 else 8
+        preps = []
+        while len(preps) < 256:
+            rpt = rng.multinomial(32, [1 / modes] * modes)
+            mu = (rng.normal(size=modes) + 1j * rng.normal(size=modes)) if loop else None
+            p = B.prepare_repeated(sym(modes), rpt, mu=mu, loop=bool(loop))
+            if p.on_device and (2 * len(p.job.edge_reps) + 7) // 8 == T:
+                preps.append(p)
+        return preps
+    # general kernel: repeated rows past the repeated-row kernel's h <= 16, 8 modes, 40 photons
     preps = []
     for _ in range(256):
-        rpt = rng.multinomial(32, [1 / 8] * 8)
+        rpt = rng.multinomial(40, [1 / 8] * 8)
         mu = (rng.normal(size=8) + 1j * rng.normal(size=8)) if loop else None
         preps.append(B.prepare_repeated(sym(8), rpt, mu=mu, loop=bool(loop)))
     return preps
@@ -112,7 +129,7 @@ def _time(preps, reps, torch, stream):
         db.close()
 
 
-DEFAULT_BLOCKS = {0: 1024, 1: 256}  # the library's built-in targets (all-ones, general)
+DEFAULT_BLOCKS = {0: 1024, 1: 256, 2: 64}  # the library's built-in targets (all-ones, general, repeated-row)
 
 
 def choose(kind, times, margin=0.01):
@@ -125,7 +142,7 @@ def choose(kind, times, margin=0.01):
     return best
 
 
-def autotune(path=DEFAULT_PATH, candidates=CANDIDATES, reps=3, classes=None, seed=1234, log=print):
+def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234, log=print):
     """Times every candidate per kernel class on the current device, applies the choice
     (``choose``) and writes the table to ``path`` (None: do not write).  Returns the entries."""
     import torch
@@ -133,13 +150,14 @@ def autotune(path=DEFAULT_PATH, candidates=CANDIDATES, reps=3, classes=None, see
     L = _native.load()
     rng = np.random.default_rng(seed)
     if classes is None:
-        classes = [(0, T, loop) for T in range(1, 7) for loop in (0, 1)] + [(1, 0, 0), (1, 0, 1)]
+        classes = ([(0, T, loop) for T in range(1, 7) for loop in (0, 1)] + [(1, 0, 0), (1, 0, 1)]
+                   + [(2, T, loop) for T in (1, 2) for loop in (0, 1)])
     stream = torch.cuda.Stream()
     entries = []
     for kind, T, loop in classes:
         preps = _class_workload(kind, T, loop, rng)
         times = {}
-        for c in candidates:
+        for c in (candidates or KIND_CANDIDATES[kind]):
             _native.check(L.wh_tune_set(kind, T, loop, int(c)))
             times[c] = _time(preps, reps, torch, stream)
         best = choose(kind, times)
@@ -150,7 +168,8 @@ def autotune(path=DEFAULT_PATH, candidates=CANDIDATES, reps=3, classes=None, see
         if log:
             log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s" % (kind, T, loop, best, e["ms"]))
     if path is not None:
-        doc = {"device": torch.cuda.get_device_name(), "candidates": list(candidates), "entries": entries}
+        doc = {"device": torch.cuda.get_device_name(),
+               "candidates": {str(k): list(candidates or v) for k, v in KIND_CANDIDATES.items()}, "entries": entries}
         with open(path, "w", encoding="utf-8") as f:
             json.dump(doc, f, indent=1)
     return entries
