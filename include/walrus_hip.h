@@ -167,6 +167,24 @@ int wh_batch_write_jobs(const wh_batch* batch, void* host_dst);
 int wh_batch_launch(const wh_batch* batch, const void* h_jobs, void* d_jobs, const double* dpool,
                     double* d_partials, double* dout, void* stream);
 
+/* Allocation-free form of wh_batch_launch.  No counterpart in the reference (its numba kernels
+ * allocate per call); it is for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The job table and the partial
+ * sums live in a static per-device scratch array of WH_NOALLOC_JOB_BYTES + 32 B x
+ * WH_NOALLOC_PARTIAL_SLOTS (16 MiB), which each GPU the library runs on holds until the process
+ * ends; a batch whose job table or partial slots exceed it fails with WH_EINVAL and nothing is
+ * enqueued.  h_jobs: the job table written by wh_batch_write_jobs to pinned host memory,
+ * unchanged until the stream passes this point; it is copied into the scratch array on `stream`,
+ * then the sieve kernels and the fold run as in wh_batch_launch (the same kernels and chunking,
+ * so the same bits).  Calls on one device share its scratch array: calls that overlap on one
+ * device (other streams, or graphs replayed concurrently) race.  The call makes no hipMalloc,
+ * hipFree or other allocating call; call wh_init(device) once before capturing it in a graph
+ * (it resolves the array's device address). */
+#define WH_NOALLOC_JOB_BYTES (1 << 20) /* job-table share: 4681 requests of 224 B */
+#define WH_NOALLOC_PARTIAL_SLOTS 491520 /* partial-sum slots: 15 MiB */
+int wh_batch_launch_noalloc(const wh_batch* batch, const void* h_jobs, const double* dpool, double* dout,
+                            void* stream);
+
 /* The serving loop's form of write_jobs + launch, one call per batch (no counterpart in the
  * reference: its requests run one numba call each).  Checks the arenas (job table bytes in
  * h_jobs / d_jobs, pool_elems complex128 in d_pool, partial slots, n results), then writes the

@@ -46,6 +46,7 @@ EXPORTS = [
     "wh_batch_info",
     "wh_batch_write_jobs",
     "wh_batch_launch",
+    "wh_batch_launch_noalloc",
     "wh_batch_submit",
     "wh_request_partials",
     "wh_server_create",
@@ -132,6 +133,8 @@ def load(path=None):
         L.wh_batch_write_jobs.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.wh_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.wh_batch_launch_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]
         L.wh_batch_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,

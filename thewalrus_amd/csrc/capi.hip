@@ -113,6 +113,34 @@ int ensure_partials(Context& c, size_t n) {
     return WH_OK;
 }
 
+// ---- Allocation-free batch launch.  The job table and the partial sums of
+// wh_batch_launch_noalloc live in `scratch`, a static device array: one copy per GPU the library
+// runs on, held until the process ends.  In host code `scratch` names a host-side stand-in, not
+// device memory, so every host use goes through the device address hipGetSymbolAddress returns,
+// resolved once per device (with that device current) and kept in g_scratch.
+constexpr size_t kScratchJobBytes = (size_t)WH_NOALLOC_JOB_BYTES;
+constexpr int64_t kScratchSlots = WH_NOALLOC_PARTIAL_SLOTS;
+constexpr size_t kScratchBytes = kScratchJobBytes + (size_t)kScratchSlots * sizeof(double4);
+static_assert(kScratchJobBytes % 256 == 0, "partials must stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev` (resolved on first use).
+int scratch_address(int dev, void** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(WH_EDEVICE, "no device address for the scratch array");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return WH_OK;
+}
+
 // Validates one request and fills the device descriptor (without chunking).
 // _hafnian.py:430-438 and :531-538 define N and the number of terms.
 int describe(const wh_request& r, wh_job_dev& j) {
@@ -578,7 +606,10 @@ int wh_init(int device) {
         return fail(WH_EINVAL, "device %d not present (count %d)", device, count);
     HIP_TRY(hipSetDevice(device));
     Context* c;
-    return ensure_context(device, &c);
+    int rc = ensure_context(device, &c);
+    if (rc) return rc;
+    void* scratch_dev = nullptr;  // resolve the no-allocation scratch before any graph capture
+    return scratch_address(device, &scratch_dev);
 }
 
 int wh_batch_create(const wh_request* reqs, int64_t n, int64_t chunk_terms, wh_batch** batch) {
@@ -617,6 +648,31 @@ int wh_batch_launch(const wh_batch* b, const void* h_jobs, void* d_jobs, const d
         HIP_TRY(hipMemcpyAsync(d_jobs, h_jobs, b->layout.jobs.size() * sizeof(wh_job_dev),
                                hipMemcpyHostToDevice, s));
     HIP_TRY(run_layout(b->layout, (const wh_job_dev*)d_jobs, dpool, (double4*)d_partials, dout, s));
+    return WH_OK;
+}
+
+int wh_batch_launch_noalloc(const wh_batch* b, const void* h_jobs, const double* dpool, double* dout,
+                            void* stream) {
+    if (!b || !h_jobs || !dpool || !dout) return fail(WH_EINVAL, "null batch or buffer");
+    const Layout& L = b->layout;
+    const size_t jb = L.jobs.size() * sizeof(wh_job_dev);
+    if (jb > kScratchJobBytes || L.blocks > kScratchSlots)
+        return fail(WH_EINVAL,
+                    "batch exceeds the static scratch (job table %zu of %zu B, %lld of %lld partial slots)", jb,
+                    kScratchJobBytes, (long long)L.blocks, (long long)kScratchSlots);
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    void* base = nullptr;
+    rc = scratch_address(dev, &base);
+    if (rc) return rc;
+    // Nothing here has to start at zero: the job table is copied in first, and every partial slot
+    // the fold reads is written by one workgroup of this launch's sieve kernels before that.
+    wh_job_dev* d_jobs = (wh_job_dev*)base;
+    double4* d_partials = (double4*)((unsigned char*)base + kScratchJobBytes);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, s));
+    HIP_TRY(run_layout(L, d_jobs, dpool, d_partials, dout, s));
     return WH_OK;
 }
 
