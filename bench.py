@@ -445,6 +445,41 @@ def dominant_kernel(step0, B, torch, stream):
     return roof
 
 
+def kernel_probe(what, B, torch, stream, reps=5):
+    """Launch sequences for kernel traces: the repeated-row batch of configuration 4 on the
+    default route and on the general kernel (config4), or the first 1000 requests of the stream as
+    batches of 500 with wh_tune_persistent 0, 1 and 2 (persistent); wall seconds per pass."""
+    from thewalrus_amd import _native
+
+    out = {}
+    if what == "config4":
+        preps = [B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in W.config4()]
+        vals = {}
+        for name, kw in (("default", {}), ("general_kernel", {"force_generic": True})):
+            b = B.DeviceBatch(preps, **kw)
+            b.run(stream)
+            ev, wall = timed_runs(b, reps, torch, stream)
+            out[name] = {"kernel_ms": ev / reps * 1e3, "requests_per_s": reps * len(preps) / wall}
+            vals[name] = b.results()
+            b.close()
+        out["max_rel_diff"] = max_rel(vals["default"], vals["general_kernel"])
+        return out
+    L = _native.load()
+    prev = L.wh_tune_persistent(-1)
+    reqs = W.config5(count=1000, seed=141)
+    try:
+        sums = {}
+        for mode in (0, 1, 2):
+            L.wh_tune_persistent(mode)
+            wall, vals = batched_steps(reqs, 500, B, torch, stream, reps=2)
+            out["mode_%d_s_per_1000" % mode] = wall
+            sums[mode] = vals
+        out["bit_identical"] = bool(np.array_equal(sums[0], sums[1]) and np.array_equal(sums[0], sums[2]))
+    finally:
+        L.wh_tune_persistent(prev)
+    return out
+
+
 def max_rel(a, b):
     return float(max(abs(x - y) / max(abs(y), 1e-300) for x, y in zip(a, b)))
 
@@ -463,6 +498,10 @@ def main():
     ap.add_argument("--no-next", action="store_true", help="skip the batched loop hafnian / montrealer lines")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the dominant kernel's launches (for counter runs under rocprofv3)")
+    ap.add_argument("--only", choices=["config4", "persistent"],
+                    help="for kernel traces under rocprofv3: config4 = the repeated-row batch on the "
+                    "default route and on the general kernel; persistent = 1000 stream requests as "
+                    "batches of 500 with the persistent launch off, on (default rule) and forced")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -488,6 +527,9 @@ def main():
     stream = torch.cuda.Stream()  # default priority; kernels and events share it
     if args.roofline_only:
         print(json.dumps({"roofline": dominant_kernel(W.config5(count=R, seed=141), B, torch, stream)}))
+        return
+    if args.only:
+        print(json.dumps(kernel_probe(args.only, B, torch, stream)))
         return
     warm = W.config5(count=max(args.warmup, 1) * R, seed=142)
     reqs = W.config5(count=K * R, seed=141)

@@ -320,14 +320,14 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
 int64_t wh_tune_get(int kind, int T, int loop);
 
 /* How plans and the serving engine launch a batch's complex all-ones requests with n in [26, 48]
- * (the symmetric matrix-core kernel's tile sides 4..6): 0 = one launch per kernel class, then the
- * fold; 1 (default) = one persistent launch for all of them (workgroups take the classes' term
- * chunks from a device queue, largest class first, and the last workgroup folds every request of
- * the batch) when the batch has two such classes or more; 2 = persistent whenever it has one.
- * The batch's other classes have their own launches first.  Results are bit-identical either
- * way (the same chunks, the same per-chunk work, the same fold order).  No counterpart in the
- * reference (numba runs one prange per call).  Returns the previous mode; mode < 0 only returns
- * the current one. */
+ * (the symmetric matrix-core kernel's tile sides 4..6): 0 (default) = one launch per kernel
+ * class, then the fold; 1 = one persistent launch for all of them (workgroups take the classes'
+ * term chunks from a device queue, largest class first, and the last workgroup folds every
+ * request of the batch) when the batch has two such classes or more; 2 = persistent whenever it
+ * has one.  The batch's other classes have their own launches first.  Results are bit-identical
+ * either way (the same chunks, the same per-chunk work, the same fold order).  No counterpart in
+ * the reference (numba runs one prange per call).  Returns the previous mode; mode < 0 only
+ * returns the current one. */
 int wh_tune_persistent(int mode);
 
 #ifdef __cplusplus

@@ -99,6 +99,32 @@ __device__ __forceinline__ E coeff_top_e(const cplx* pt, const cplx* lv, int h) 
     return res;
 }
 
+// The same coefficient with b_i kept in the term's slot (written over p_i) instead of registers:
+// about half the registers of coeff_top_e, for the repeated-row kernel's H = 16.
+template <class E, int H, bool LOOP>
+__device__ __forceinline__ E coeff_top_lds(cplx* pt, const cplx* lv, int h) {
+#pragma unroll
+    for (int i = 1; i <= H; ++i) {
+        E v = escale(eget(pt[i], (E*)nullptr), 0.5);
+        if (LOOP) v = eadd(v, escale(eget(lv[i], (E*)nullptr), 0.5 * i));
+        pt[i] = etoc(i <= h ? v : ezero_v((E*)nullptr));
+    }
+    E e[H + 1];
+    e[0] = eone((E*)nullptr);
+    E res = e[0];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        E s0 = ezero_v((E*)nullptr), s1 = ezero_v((E*)nullptr);
+#pragma unroll
+        for (int i = 1; i <= k; i += 2) emac(s0, eget(pt[i], (E*)nullptr), e[k - i]);
+#pragma unroll
+        for (int i = 2; i <= k; i += 2) emac(s1, eget(pt[i], (E*)nullptr), e[k - i]);
+        e[k] = escale(eadd(s0, s1), kInvJ[k]);
+        if (k == h) res = e[k];
+    }
+    return res;
+}
+
 template <int T> struct Lds {
     static constexpr int S = 8 * T + WH_PM1_PAD;
 };
@@ -396,37 +422,41 @@ __device__ __forceinline__ void pm1_body(const wh_job_dev& J, const cplx* __rest
                     tile_store<T>(Pw, acc, I0, J0);
                     qe = tile_tdot<T>(Pw, acc, I0, J0);
                 }
-                cplx qoc = wave_csum(etoc(qo));
-                cplx qec = wave_csum(etoc(qe));
-                const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
-                if (lane == 0) {
-                    pt[ko] = qoc;
-                    pt[ke] = qec;  // ke <= h + 1 < PTN
-                }
+                // the power's results, summed over the wave eight at once: p_ko, p_ke, v_2a, v_(2a+1)
+                E de, dodd;
+                ezero(de);
+                ezero(dodd);
+                E znew[T];
                 if (LOOP) {
-                    E y[T], z[T], xdI[T], dJ[T];
+                    E y[T], xdI[T], dJ[T];
                     load_xd(xdI);
                     load_d(dJ);
                     tile_yvec<T>(acc, xdI, y);
-                    tile_zvec<T>(acc, dJ, z);
-                    E de, dodd;
-                    ezero(de);
-                    ezero(dodd);
+                    tile_zvec<T>(acc, dJ, znew);
                     if (ti == tj) {
 #pragma unroll
                         for (int c = 0; c < T; ++c) {
                             emac(de, y[c], zprev[c]);
-                            emac(dodd, y[c], z[c]);
+                            emac(dodd, y[c], znew[c]);
                         }
                     }
-                    cplx vec_ = wave_csum(etoc(de));
-                    cplx voc_ = wave_csum(etoc(dodd));
-                    if (lane == 0) {
-                        if (2 * a <= h) lv[2 * a] = vec_;
-                        if (2 * a + 1 <= h) lv[2 * a + 1] = voc_;
-                    }
+                }
+                const cplx qoc = etoc(qo), qec = etoc(qe), dec = etoc(de), doc = etoc(dodd);
+                const double v8[8] = {qoc.re, qoc.im, qec.re, qec.im, dec.re, dec.im, doc.re, doc.im};
+                const double sum8 = wave_sum8(v8, lane);
+                const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+                if ((lane & 7) == 0) {  // lane 8k holds the sum of v8[k]
+                    const int k = lane >> 3;
+                    double* dst = nullptr;
+                    if (k < 2) dst = &pt[ko].re;
+                    else if (k < 4) dst = &pt[ke].re;  // ke <= h + 1 < PTN
+                    else if (LOOP && k < 6 && 2 * a <= h) dst = &lv[2 * a].re;
+                    else if (LOOP && k >= 6 && 2 * a + 1 <= h) dst = &lv[2 * a + 1].re;
+                    if (dst) dst[k & 1] = sum8;
+                }
+                if (LOOP) {
 #pragma unroll
-                    for (int r = 0; r < T; ++r) zprev[r] = z[r];
+                    for (int r = 0; r < T; ++r) zprev[r] = znew[r];
                 }
             }
         }
@@ -660,37 +690,41 @@ __device__ __forceinline__ void rep_body(const wh_job_dev& J, const cplx* __rest
                 tile_store<T>(Pw, acc, I0, J0);
                 qe = tile_tdot<T>(Pw, acc, I0, J0);
             }
-            cplx qoc = wave_csum(etoc(qo));
-            cplx qec = wave_csum(etoc(qe));
-            const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
-            if (lane == 0) {
-                pt[ko] = qoc;
-                pt[ke] = qec;  // ke <= h + 1 < PTN
-            }
+            // the power's results, summed over the wave eight at once: p_ko, p_ke, v_2a, v_(2a+1)
+            E de, dodd;
+            ezero(de);
+            ezero(dodd);
+            E znew[T];
             if (LOOP) {
-                E y[T], z[T], xdI[T], dJ[T];
+                E y[T], xdI[T], dJ[T];
                 load_xd(xdI);
                 load_d(dJ);
                 tile_yvec<T>(acc, xdI, y);
-                tile_zvec<T>(acc, dJ, z);
-                E de, dodd;
-                ezero(de);
-                ezero(dodd);
+                tile_zvec<T>(acc, dJ, znew);
                 if (ti == tj) {
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         emac(de, y[c], zprev[c]);
-                        emac(dodd, y[c], z[c]);
+                        emac(dodd, y[c], znew[c]);
                     }
                 }
-                cplx vec_ = wave_csum(etoc(de));
-                cplx voc_ = wave_csum(etoc(dodd));
-                if (lane == 0) {
-                    if (2 * a <= h) lv[2 * a] = vec_;
-                    if (2 * a + 1 <= h) lv[2 * a + 1] = voc_;
-                }
+            }
+            const cplx qoc = etoc(qo), qec = etoc(qe), dec = etoc(de), doc = etoc(dodd);
+            const double v8[8] = {qoc.re, qoc.im, qec.re, qec.im, dec.re, dec.im, doc.re, doc.im};
+            const double sum8 = wave_sum8(v8, lane);
+            const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+            if ((lane & 7) == 0) {  // lane 8k holds the sum of v8[k]
+                const int k = lane >> 3;
+                double* dst = nullptr;
+                if (k < 2) dst = &pt[ko].re;
+                else if (k < 4) dst = &pt[ke].re;  // ke <= h + 1 < PTN
+                else if (LOOP && k < 6 && 2 * a <= h) dst = &lv[2 * a].re;
+                else if (LOOP && k >= 6 && 2 * a + 1 <= h) dst = &lv[2 * a + 1].re;
+                if (dst) dst[k & 1] = sum8;
+            }
+            if (LOOP) {
 #pragma unroll
-                for (int r = 0; r < T; ++r) zprev[r] = z[r];
+                for (int r = 0; r < T; ++r) zprev[r] = znew[r];
             }
         }
         if (LOOP) {
@@ -709,8 +743,8 @@ __device__ __forceinline__ void rep_body(const wh_job_dev& J, const cplx* __rest
         if (slot == NB || t + nw >= t1) {
             wave_lds_sync();
             if (lane < slot) {
-                const cplx* ptj = slots + lane * 2 * PTN;
-                const E top = coeff_top_e<E, H, LOOP>(ptj, ptj + PTN, h);
+                cplx* ptj = slots + lane * 2 * PTN;
+                const E top = coeff_top_lds<E, H, LOOP>(ptj, ptj + PTN, h);
                 acc_sum.add(cmulr(etoc(top), ptj[0].re));
             }
             wave_lds_sync();  // the slots are free again
@@ -812,8 +846,13 @@ hipError_t wh_launch_pm1(int T, bool half, bool real, bool loop, const wh_job_de
 }
 
 
+// Four waves per SIMD: the complex H = 16 instances then keep part of the coefficient state in
+// scratch (about 190 bytes per lane) instead of running two or three waves per SIMD.
+#ifndef WH_REP_WAVES_PER_EU
+#define WH_REP_WAVES_PER_EU 4
+#endif
 template <int T, bool REAL, bool LOOP, int H>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WH_REP_WAVES_PER_EU)))
 wh_sieve_rep(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
              double4* __restrict__ partials, int64_t block_offset) {
     extern __shared__ __align__(16) unsigned char wh_smem[];

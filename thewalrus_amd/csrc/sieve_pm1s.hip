@@ -68,17 +68,6 @@ __device__ __forceinline__ cplx cflip(cplx v, uint32_t bit) { return cmk(sflip(v
 // the partner of index c in the other half (identity on the padding)
 __device__ __forceinline__ int swp(int c, int h) { return c < h ? c + h : (c < 2 * h ? c - h : c); }
 
-__device__ __forceinline__ double dswap32(double a, double b) {
-    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
-    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
-    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
-}
-__device__ __forceinline__ double dswap16(double a, double b) {
-    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
-    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
-    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
-}
-
 template <int T> struct GeoS {
     static constexpr int NP = 8 * T;
     static constexpr int S = NP + WH_PM1S_PAD;

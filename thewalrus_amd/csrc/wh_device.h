@@ -174,6 +174,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 __device__ __forceinline__ cplx wave_csum(cplx z) { return cmk(wave_sum(z.re), wave_sum(z.im)); }
 
+__device__ __forceinline__ double dswap32(double a, double b) {  // lower half: a + a^32; upper: b + b^32
+    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ double dswap16(double a, double b) {  // even rows: a + a^16; odd rows: b + b^16
+    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Eight wave sums at once: lane l returns the sum over all lanes of v[(l >> 3) & 7].  Each
+// butterfly stage halves the number of values a lane carries (the swaps move two values per
+// instruction pair), 34 instructions where eight separate wave sums take about 140.
+__device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
+    double s1[4], s2[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s1[k] = dswap32(v[k], v[k + 4]);  // value k + 4 bit5
+#pragma unroll
+    for (int k = 0; k < 2; ++k) s2[k] = dswap16(s1[k], s1[k + 2]);  // value k + 2 bit4 + 4 bit5
+    const bool b3 = (lane >> 3) & 1;
+    double keep = b3 ? s2[1] : s2[0], send = b3 ? s2[0] : s2[1];
+    double s3 = keep + dpp_d<WH_DPP_ROR8>(send);  // value bit3 + 2 bit4 + 4 bit5
+    return group8_sum(s3);
+}
+
 // Lanes of one wave exchanging data through LDS: the hardware completes a wave's LDS operations
 // in issue order, but the compiler reasons per lane and may move a load above another lane's
 // store (or forward a conditional store to its own lane only).  The wavefront-scope fences keep
