@@ -355,13 +355,15 @@ def w_rep(job):
     return float((8.0 * m ** 3 * np.maximum(np.minimum(h + 1, m) - 2, 0)).sum())
 
 
-def mfma_flops_executed(n):
-    """FP64 flops the symmetric-power kernel issues on the matrix cores for one n x n request:
-    per product T(T+1)/2 block groups of 8 x 8, each 4 real 4x4x4 (four-block) MFMAs per k chunk
-    of 4, 2T chunks (n_pad = 8T); ceil(h/2) - 1 products per term, 2^(h-1) terms."""
+def mfma_flops_executed(n, loop):
+    """FP64 flops the symmetric-power kernel issues on the matrix cores for one complex n x n
+    request: per product T(T+1)/2 block groups of 8 x 8, each 3 real 4x4x4 (four-block) MFMAs
+    per k chunk of 4 (Gauss's three-multiplication form; 4 for the T = 6 loop and T >= 7
+    instances), 2T chunks (n_pad = 8T); ceil(h/2) - 1 products per term, 2^(h-1) terms."""
     h = n // 2
     T = (n + 7) // 8
-    per_product = T * (T + 1) // 2 * 4 * 512 * 2 * T
+    mults = 3 if T <= 5 or (T == 6 and not loop) else 4
+    per_product = T * (T + 1) // 2 * mults * 512 * 2 * T
     return 2.0 ** (h - 1) * ((h + 1) // 2 - 1) * per_product
 
 
@@ -419,7 +421,7 @@ def dominant_kernel(step0, B, torch, stream):
         sub.run(stream)
         t, _ = timed_runs(sub, 1, torch, stream)
         wref = sum(W.w_glynn(step0[i][0].shape[0]) for i in idx)
-        wexe = sum(mfma_flops_executed(step0[i][0].shape[0]) for i in idx)
+        wexe = sum(mfma_flops_executed(step0[i][0].shape[0], step0[i][1]) for i in idx)
         if best is None or t > best[0]:
             best = (t, sub.flops, want_loop, len(idx), wref, wexe)
         sub.close()
@@ -435,8 +437,9 @@ def dominant_kernel(step0, B, torch, stream):
             "kernel": "wh_sieve_pm1s<5,%s> (FP64 matrix cores, symmetric powers, n in [34,40], %d requests of "
             "the first step)" % ("loop" if lp else "plain", cnt),
             "flops_formula": "sum over requests of 2^(h-1) * 8 * n(n+1)/2 * n * (ceil(h/2)-1): the upper "
-            "triangle of the symmetric complex n x n products per term (powers up to ceil(h/2)); "
-            "lower-order trace and loop terms not counted",
+            "triangle of the symmetric complex n x n products per term (powers up to ceil(h/2)), 8 flops "
+            "per complex multiply-add as in the standard product (the kernel forms it with 3 real "
+            "multiplications: mfma_executed_tflops); lower-order trace and loop terms not counted",
             "mfma_executed_tflops": wexe / t / 1e12,
             "kernel_time_s": t,
             "reference_algorithm_equivalent_tflops": wref / t / 1e12,

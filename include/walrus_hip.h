@@ -319,6 +319,15 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
 /* The table entry, or -1 for an entry out of range. */
 int64_t wh_tune_get(int kind, int T, int loop);
 
+/* Waves per workgroup for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1, 2)
+ * kernel class, 1..4, or 0 for the default (the count that fits the most waves per CU in the
+ * LDS); a value whose workgroup would not fit the LDS is ignored.  Set by the autotuner; results
+ * change in their last bits at most (the order in which a chunk's terms are summed). */
+int wh_tune_set_waves(int kind, int T, int loop, int waves);
+
+/* The waves entry, or -1 for an entry out of range. */
+int wh_tune_get_waves(int kind, int T, int loop);
+
 /* How plans and the serving engine launch a batch's complex all-ones requests with n in [26, 48]
  * (the symmetric matrix-core kernel's tile sides 4..6): 0 (default) = one launch per kernel
  * class, then the fold; 1 = one persistent launch for all of them (workgroups take the classes'

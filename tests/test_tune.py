@@ -68,3 +68,24 @@ def test_committed_table_is_valid():
     assert keys == set(tune.current().keys())
     for e in doc["entries"]:
         assert e["blocks_per_request"] == tune.choose(e["kind"], {int(k): v for k, v in e["ms"].items()})
+        if "waves_ms" in e:
+            assert e["waves"] == tune.choose_waves({int(k): v for k, v in e["waves_ms"].items()})
+
+
+def test_waves_round_trip_and_range(restore_table):
+    L = _native.load()
+    before = tune.current_waves()
+    try:
+        assert L.wh_tune_set_waves(0, 5, 1, 2) == 0 and L.wh_tune_get_waves(0, 5, 1) == 2
+        assert L.wh_tune_set_waves(2, 2, 0, 4) == 0 and L.wh_tune_get_waves(2, 2, 0) == 4
+        for args in ((1, 0, 0, 1), (0, 7, 0, 1), (0, 3, 0, 5), (2, 3, 0, 1), (0, 3, 0, -1)):
+            assert L.wh_tune_set_waves(*args) != 0
+        assert L.wh_tune_get_waves(1, 0, 0) == -1
+    finally:
+        for (k, T, lp), w in before.items():
+            L.wh_tune_set_waves(k, T, lp, w)
+
+
+def test_choose_waves_keeps_default_within_noise():
+    assert tune.choose_waves({0: 10.0, 1: 9.95, 2: 12.0}) == 0
+    assert tune.choose_waves({0: 10.0, 2: 9.5}) == 2

@@ -71,6 +71,8 @@ EXPORTS = [
     "wh_graph_destroy",
     "wh_tune_set",
     "wh_tune_get",
+    "wh_tune_set_waves",
+    "wh_tune_get_waves",
     "wh_tune_persistent",
     "wh_loop_hafnian_batch",
     "wh_loop_hafnian_batch_gamma",
@@ -127,6 +129,8 @@ def load(path=None):
         L.wh_tune_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_tune_get.restype = ctypes.c_int64
         L.wh_tune_persistent.argtypes = [ctypes.c_int]
+        L.wh_tune_set_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.wh_tune_get_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_void_p)]
         L.wh_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _ip, _ip, _dp]

@@ -232,12 +232,20 @@ struct GroupKey {
     }
 };
 
-// Complex inputs with the half-products formulation run the products on the matrix cores
-// unless WH_OPT_VALU asks for the register-tiled vector kernel: hafnians with the symmetric
-// powers C_a = B^a X D_d (KIND_PM1S, upper block groups only), the montrealer (whose 0/1
-// factors break that symmetry) and WH_OPT_NONSYM requests with the full powers (KIND_PM1M).
+// Inputs with the half-products formulation run the products on the matrix cores unless
+// WH_OPT_VALU asks for the register-tiled vector kernel: hafnians with the symmetric powers
+// C_a = B^a X D_d (KIND_PM1S, upper block groups only; real ones up to T = 5 with one MFMA per
+// block group),
+// the complex montrealer (whose 0/1 factors break that symmetry) and complex WH_OPT_NONSYM
+// requests with the full powers (KIND_PM1M); the real montrealer and real WH_OPT_NONSYM requests
+// stay on the vector kernel.
 static void pick_matrix_cores(GroupKey& g, int options) {
-    if (g.kind != KIND_PM1 || g.real || !g.half || (options & WH_OPT_VALU)) return;
+    if (g.kind != KIND_PM1 || !g.half || (options & WH_OPT_VALU)) return;
+    if (g.real) {  // real hafnians up to n = 40: the symmetric kernel with one MFMA per block group
+        // (at T = 6 its real instance keeps state in scratch: the vector kernel stays)
+        if (!g.mtl && !(options & WH_OPT_NONSYM) && g.T <= 5) g.kind = KIND_PM1S;
+        return;
+    }
     const bool full = g.mtl || (options & WH_OPT_NONSYM);
     // full powers at T = 6 (n in [42, 48]) keep 72 complex accumulators per lane and spill: the
     // vector kernel keeps them; the symmetric kernel's 21 upper groups fit
@@ -367,11 +375,20 @@ int pm1_waves(int T, bool real, int kind) {
 // defaults below): [0][T][loop] all-ones kernels by tile side class, [1][0][loop] general kernel,
 // [2][T][loop] repeated-row kernel (T = 1, 2).
 std::atomic<int64_t> g_tune_blocks[3][kPm1MaxT + 1][2];
+// Waves per workgroup per kernel class (wh_tune_set_waves; 0 = the LDS-based choice), same indexing.
+std::atomic<int> g_tune_waves[3][kPm1MaxT + 1][2];
 
 int64_t tuned_blocks(const GroupKey& g) {
     const int kind = all_ones_kind(g.kind) ? 0 : g.kind == KIND_REP ? 2 : 1;
     const int T = kind == 1 ? 0 : std::min(std::max(g.T, 0), kPm1MaxT);
     return g_tune_blocks[kind][T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
+}
+
+int tuned_waves(const GroupKey& g) {
+    if (!(all_ones_kind(g.kind) || g.kind == KIND_REP)) return 0;
+    const int kind = all_ones_kind(g.kind) ? 0 : 2;
+    const int T = std::min(std::max(g.T, 0), kPm1MaxT);
+    return g_tune_waves[kind][T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
 }
 
 int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
@@ -410,7 +427,7 @@ double term_flops(const wh_job_dev& j, const GroupKey& g) {
     if (g.kind == KIND_GENERIC && !(j.flags & WH_F_CHARPOLY)) np = h - 1;  // every power formed
     if ((all_ones_kind(g.kind) || g.kind == KIND_REP) && g.half) np = (h + 1) / 2 - 1;
     if (np < 0) np = 0;
-    double c = ((g.kind == KIND_PM1 || g.kind == KIND_REP) && g.real) ? 2.0 : 8.0;
+    double c = ((g.kind == KIND_PM1 || g.kind == KIND_REP || g.kind == KIND_PM1S) && g.real) ? 2.0 : 8.0;
     if (g.kind == KIND_PM1S) return c * (m * (m + 1.0) / 2.0) * m * np;
     return c * m * (double)m * m * np;
 }
@@ -473,10 +490,12 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
             gr.key = g;
             gr.job_begin = k;
             gr.block_begin = base;
-            gr.nw = g.nw                  ? g.nw
-                    : all_ones_kind(g.kind) ? pm1_waves(g.T, g.real != 0, g.kind)
-                    : g.kind == KIND_REP  ? rep_waves(g.T, g.real != 0, g.H)
-                                          : 1;
+            const int tw = tuned_waves(g);  // the tuning table's choice, where it fits the LDS
+            gr.nw = g.nw                                                            ? g.nw
+                    : (tw >= 1 && tw <= 4 && group_lds_bytes(g, tw) <= 160 * 1024) ? tw
+                    : all_ones_kind(g.kind)                                         ? pm1_waves(g.T, g.real != 0, g.kind)
+                    : g.kind == KIND_REP                                            ? rep_waves(g.T, g.real != 0, g.H)
+                                                                                    : 1;
             L.groups.push_back(gr);
         }
         Group& gr = L.groups.back();
@@ -522,7 +541,7 @@ int device_cus(int dev) {
     return cus;
 }
 
-bool mixed_group(const GroupKey& k) { return k.kind == KIND_PM1S && k.T >= 4 && k.T <= 6; }
+bool mixed_group(const GroupKey& k) { return k.kind == KIND_PM1S && !k.real && k.T >= 4 && k.T <= 6; }
 
 // Enqueues the sieve kernels of every group and the fold.
 hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* pool, double4* partials,
@@ -543,7 +562,8 @@ hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* poo
         int64_t nb = g.block_end - g.block_begin;
         hipError_t e;
         if (g.key.kind == KIND_PM1S)
-            e = wh_launch_pm1s(g.key.T, g.key.loop != 0, gj, nj, pool, partials, nb, g.block_begin, g.nw, s);
+            e = wh_launch_pm1s(g.key.T, g.key.loop != 0, g.key.real != 0, gj, nj, pool, partials, nb, g.block_begin,
+                               g.nw, s);
         else if (g.key.kind == KIND_PM1M)
             e = wh_launch_pm1m(g.key.T, g.key.loop != 0, g.key.mtl != 0, gj, nj, pool, partials, nb, g.block_begin,
                                g.nw, s);
@@ -849,6 +869,21 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request) {
                     (long long)blocks_per_request);
     g_tune_blocks[kind][T][loop].store(blocks_per_request, std::memory_order_relaxed);
     return WH_OK;
+}
+
+int wh_tune_set_waves(int kind, int T, int loop, int waves) {
+    if (kind < 0 || kind > 2 || kind == 1 || loop < 0 || loop > 1 || waves < 0 || waves > 4 ||
+        (kind == 0 && (T < 1 || T > kPm1MaxT)) || (kind == 2 && (T < 1 || T > kRepMaxT)))
+        return fail(WH_EINVAL, "waves entry out of range (kind %d, T %d, loop %d, waves %d)", kind, T, loop, waves);
+    g_tune_waves[kind][T][loop].store(waves, std::memory_order_relaxed);
+    return WH_OK;
+}
+
+int wh_tune_get_waves(int kind, int T, int loop) {
+    if (kind < 0 || kind > 2 || kind == 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
+        (kind == 2 && (T < 1 || T > kRepMaxT)))
+        return -1;
+    return g_tune_waves[kind][T][loop].load(std::memory_order_relaxed);
 }
 
 int wh_tune_persistent(int mode) {

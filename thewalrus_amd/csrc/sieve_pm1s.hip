@@ -1,7 +1,8 @@
 // sieve_pm1s.hip -- the all-ones Glynn sieve on the FP64 matrix cores with symmetric powers:
 // the terms of _hafnian.py:443-462 (plain) and :546-569 (even loop) when rpt is all ones,
-// complex matrices, half-products formulation, with 60% of the matrix-core work of
-// sieve_pm1m.hip at n_pad = 40.
+// complex matrices (and real ones, one real MFMA per block product), half-products
+// formulation, with 60% of the block products of sieve_pm1m.hip at n_pad = 40, each complex one
+// from three real MFMAs.
 //
 // With A = Ax symmetric, X the swap of the two halves and D_d = diag(d) (d = +-1 repeating over
 // the halves, so X D_d = D_d X and (X D_d)^2 = I), the term matrix is B = A X D_d and
@@ -22,7 +23,8 @@
 // Lane map and MFMA layout as sieve_pm1m.hip: x = lane & 3, br = bit 2, bc = bit 3,
 // y = lane >> 4; group (p, q) holds R[8p + 4br + y][8q + 4bc + x].  The signs d of the lane's
 // rows and columns live in one mask register and flip the sign bit of the values they scale;
-// the product's -Im(a) Im(b) term uses the matrix core's negate modifier on B.
+// where a complex product takes four MFMAs, its -Im(a) Im(b) term uses the matrix core's negate
+// modifier on B.
 #include <algorithm>
 
 #include "wh_device.h"
@@ -32,6 +34,11 @@ namespace {
 
 #ifndef WH_PM1S_PAD
 #define WH_PM1S_PAD 2
+#endif
+// complex products with three real MFMAs per block group and k chunk instead of four (below);
+// 0 turns the form off
+#ifndef WH_PM1S_3M
+#define WH_PM1S_3M 1
 #endif
 
 static __constant__ double kInvJs[WH_EMAX + 1] = {
@@ -289,8 +296,23 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
 
 // Rn = Vb (X A) for the upper groups (fully unrolled k chunks) with power a_prev's epilogue
 // (from Ro and Vb = Pw = V_(a_prev)) between the MFMA pairs; operands of chunk kc + 1 load
-// while chunk kc runs.
-template <int T, bool LOOP>
+// while chunk kc runs.  RE: matrix and loop weights are real (every imaginary part zero, also in
+// LDS): one MFMA per group and chunk (Re x Re) instead of four, the imaginary accumulators stay
+// zero, and only the real halves of the operands are read.
+// Complex operands (WH_PM1S_3M): Gauss's three-multiplication form.  With X = Re a Re b,
+// Y = Im a Im b and Z = (Re a + Im a)(Re b + Im b) summed over k, Re(ab) = X - Y and
+// Im(ab) = Z - X - Y: three MFMAs per group and chunk and 2T operand sums per chunk instead of
+// four MFMAs; the accumulators of X, Y, Z combine once at the end of the product.  Normwise the
+// rounding error is of the same order as the four-product form's; the imaginary part carries
+// it relative to |Re a||Re b| + |Im a||Im b| rather than to itself.  The extra accumulators
+// and operand sums push the T = 6 loop and T >= 7 instances further into scratch, where the
+// form measured slower (n = 42 loop 292 -> 323 ms, n = 50 5.5 -> 6.5 s): they keep four MFMAs.
+template <int T, bool LOOP, bool RE>
+constexpr bool pm1s_3m() {
+    return !RE && WH_PM1S_3M && (T <= 5 || (T == 6 && !LOOP));
+}
+
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, const cplx* __restrict__ Ms,
                                                 double (&nR)[T][T], double (&nI)[T][T], const LaneMapS& L,
                                                 uint32_t fm, const double (&oR)[T][T], const double (&oI)[T][T],
@@ -298,12 +320,15 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
                                                 cplx* zcur, uint64_t tu, int h, cplx* pt, cplx* lv) {
     constexpr int S = GeoS<T>::S;
     constexpr int NK = 2 * T;
+    constexpr bool G3 = pm1s_3m<T, LOOP, RE>();
+    double nY[T][T];  // G3: sum of Im a Im b
 #pragma unroll
     for (int p = 0; p < T; ++p)
 #pragma unroll
         for (int q = p; q < T; ++q) {
             nR[p][q] = 0.0;
             nI[p][q] = 0.0;
+            if (G3) nY[p][q] = 0.0;
         }
     EpiS<T> E;
     epis_init<T, LOOP>(E, L, zprev, fm, h);
@@ -311,9 +336,9 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
     const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
     cplx avn[T], bqn[T];
 #pragma unroll
-    for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S];
+    for (int p = 0; p < T; ++p) avn[p] = RE ? cmk(pa[8 * p * S].re, 0.0) : pa[8 * p * S];
 #pragma unroll
-    for (int q = 0; q < T; ++q) bqn[q] = pb[8 * q];
+    for (int q = 0; q < T; ++q) bqn[q] = RE ? cmk(pb[8 * q].re, 0.0) : pb[8 * q];
     epis_load_row<T, LOOP>(0, E, L, Vb, Ms, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
@@ -324,11 +349,20 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
         for (int q = 0; q < T; ++q) bq[q] = bqn[q];
         if (kc + 1 < NK) {
 #pragma unroll
-            for (int p = 0; p < T; ++p) avn[p] = pa[8 * p * S + 4 * (kc + 1)];
+            for (int p = 0; p < T; ++p)
+                avn[p] = RE ? cmk(pa[8 * p * S + 4 * (kc + 1)].re, 0.0) : pa[8 * p * S + 4 * (kc + 1)];
 #pragma unroll
-            for (int q = 0; q < T; ++q) bqn[q] = pb[4 * (kc + 1) * S + 8 * q];
+            for (int q = 0; q < T; ++q)
+                bqn[q] = RE ? cmk(pb[4 * (kc + 1) * S + 8 * q].re, 0.0) : pb[4 * (kc + 1) * S + 8 * q];
         }
         if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP>((kc + 1) >> 1, E, L, Vb, Ms, h);
+        double as[T], bs[T];
+        if (G3) {
+#pragma unroll
+            for (int p = 0; p < T; ++p) as[p] = av[p].re + av[p].im;
+#pragma unroll
+            for (int q = 0; q < T; ++q) bs[q] = bq[q].re + bq[q].im;
+        }
         const int nst = epis_steps<T, LOOP>(kc);
         int st = 0;
 #pragma unroll
@@ -336,23 +370,41 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
 #pragma unroll
             for (int q = p; q < T; ++q) {
                 nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
-                nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
+                if (G3) nY[p][q] = mfma4(av[p].im, bq[q].im, nY[p][q]);
+                else if (!RE) nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
                 if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
+        if (!RE) {
+#pragma unroll
+            for (int p = 0; p < T; ++p)
+#pragma unroll
+                for (int q = p; q < T; ++q) {
+                    if (G3) {
+                        nI[p][q] = mfma4(as[p], bs[q], nI[p][q]);
+                    } else {
+                        nR[p][q] = mfma4_nb(av[p].im, bq[q].im, nR[p][q]);
+                        nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
+                    }
+                    if (st < nst)
+                        epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                    ++st;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+        }
+        for (; st < nst; ++st)
+            epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+    }
+    if (G3) {
 #pragma unroll
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = p; q < T; ++q) {
-                nR[p][q] = mfma4_nb(av[p].im, bq[q].im, nR[p][q]);
-                nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
-                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
-                ++st;
-                __builtin_amdgcn_sched_barrier(0);
+                const double x = nR[p][q], y = nY[p][q];
+                nR[p][q] = x - y;
+                nI[p][q] = nI[p][q] - (x + y);
             }
-        for (; st < nst; ++st)
-            epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
     }
 }
 
@@ -378,7 +430,7 @@ __device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const doubl
 // One chunk (partial slot b) of job J by the first nw waves of the workgroup; any further waves
 // only take part in the staging and the barriers (the persistent form runs every class with the
 // largest class's workgroup).
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __restrict__ pool,
                                           double4* __restrict__ partials, int64_t b, unsigned char* smem, int nw) {
     constexpr int NP = GeoS<T>::NP;
@@ -471,14 +523,14 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
             int a = 2;
 #pragma unroll 1
             for (; a + 1 <= Ahalf; a += 2) {
-                sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
                 sym_tdot_load<T>(tv, Pw, L, h);  // Pw holds V_(a-1): for tr(B^(2a-1))
                 wave_lds_sync();
                 sym_store<T>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
                 qo = sym_tdot_sum<T>(tv, bR, bI, fm);
-                sym_product_epi<T, LOOP>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
+                sym_product_epi<T, LOOP, RE>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
                                          Zb + (a & 1) * NP, tu, h, pt, lv);
                 sym_tdot_load<T>(tv, Pw, L, h);
                 wave_lds_sync();
@@ -487,7 +539,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
                 qo = sym_tdot_sum<T>(tv, aR, aI, fm);
             }
             if (a <= Ahalf) {  // one power left: R_a into b, then its epilogue alone
-                sym_product_epi<T, LOOP>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
                 sym_tdot_load<T>(tv, Pw, L, h);
                 wave_lds_sync();
@@ -553,7 +605,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
 
 }  // namespace
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE>
 __global__ void __launch_bounds__(256)
 wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
               double4* __restrict__ partials, int64_t block_offset) {
@@ -564,7 +616,7 @@ wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
     }
-    pm1s_body<T, LOOP>(jobs[lo], pool, partials, b, wh_smem, blockDim.x / WH_WAVE);
+    pm1s_body<T, LOOP, RE>(jobs[lo], pool, partials, b, wh_smem, blockDim.x / WH_WAVE);
 }
 
 // ---- Persistent form for a whole batch: the chunks of the symmetric-kernel groups with tile
@@ -650,15 +702,15 @@ size_t wh_pm1s_lds_bytes(int T, int nw) {
     return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE>
 static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
     size_t lds = wh_pm1s_lds_bytes(T, nw);
     static wh_lds_limit lim;
-    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP>, lds, lim);
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lim);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((wh_sieve_pm1s<T, LOOP>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream, jobs,
-                       njobs, (const cplx*)pool, (double4*)partials, block_offset);
+    hipLaunchKernelGGL((wh_sieve_pm1s<T, LOOP, RE>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream,
+                       jobs, njobs, (const cplx*)pool, (double4*)partials, block_offset);
     return hipGetLastError();
 }
 
@@ -684,22 +736,28 @@ hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, 
     return hipGetLastError();
 }
 
-hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                          int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
-#define WH_PM1S_CASE(TT)                                                                                   \
-    case TT:                                                                                               \
-        return loop ? launch_s<TT, true>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream)  \
-                    : launch_s<TT, false>(jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+hipError_t wh_launch_pm1s(int T, bool loop, bool real, const wh_job_dev* jobs, int njobs, const void* pool,
+                          void* partials, int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+#define WH_PM1S_ARGS jobs, njobs, pool, partials, nblocks, block_offset, nw, stream
+#define WH_PM1S_CASE(TT)                                                              \
+    case TT:                                                                          \
+        return loop ? launch_s<TT, true, false>(WH_PM1S_ARGS) : launch_s<TT, false, false>(WH_PM1S_ARGS);
+#define WH_PM1S_CASE_R(TT)                                                            \
+    case TT:                                                                          \
+        if (real) return loop ? launch_s<TT, true, true>(WH_PM1S_ARGS) : launch_s<TT, false, true>(WH_PM1S_ARGS); \
+        return loop ? launch_s<TT, true, false>(WH_PM1S_ARGS) : launch_s<TT, false, false>(WH_PM1S_ARGS);
     switch (T) {
-        WH_PM1S_CASE(1)
-        WH_PM1S_CASE(2)
-        WH_PM1S_CASE(3)
-        WH_PM1S_CASE(4)
-        WH_PM1S_CASE(5)
+        WH_PM1S_CASE_R(1)
+        WH_PM1S_CASE_R(2)
+        WH_PM1S_CASE_R(3)
+        WH_PM1S_CASE_R(4)
+        WH_PM1S_CASE_R(5)
         WH_PM1S_CASE(6)
         WH_PM1S_CASE(7)
         WH_PM1S_CASE(8)
         default: return hipErrorInvalidValue;
     }
 #undef WH_PM1S_CASE
+#undef WH_PM1S_CASE_R
+#undef WH_PM1S_ARGS
 }

@@ -45,8 +45,8 @@ struct wh_mixed_group;
 hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, const wh_job_dev* jobs, int njobs,
                                      const void* pool, void* partials, unsigned* counters, void* out, int nwg,
                                      hipStream_t stream);
-hipError_t wh_launch_pm1s(int T, bool loop, const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
-                          int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream);
+hipError_t wh_launch_pm1s(int T, bool loop, bool real, const wh_job_dev* jobs, int njobs, const void* pool,
+                          void* partials, int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream);
 size_t wh_rep_lds_bytes(int T, bool real, int nw, int H);
 hipError_t wh_launch_rep(int T, int H, bool real, bool loop, const wh_job_dev* jobs, int njobs, const void* pool,
                          void* partials, int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream);

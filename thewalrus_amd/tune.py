@@ -14,6 +14,11 @@ the built-in default by more than 1% (`choose`):
 * kind 2: the repeated-row kernel (even N, 2E <= 16, h <= 16), by tile side class T = 1, 2,
   plain and loop.
 
+For the all-ones and repeated-row classes the tuner then times the waves per workgroup, 1 to 4
+against the built-in choice (the count that fits the most waves per CU in the LDS), and keeps a
+count only when it beats that choice by more than 1% (`wh_tune_set_waves`; a count whose
+workgroup does not fit the LDS is ignored by the library).
+
 The table lives in a JSON file (`tuning.json` next to this module by default, or the path in
 the environment variable ``WH_TUNING``; ``WH_TUNING=none`` keeps the built-in defaults) that the
 library applies when it is loaded.  Tuning changes only the chunking: each result changes in
@@ -46,10 +51,13 @@ def table_path():
 
 
 def apply(entries, lib=None):
-    """Sets the library's table from a list of {"kind", "T", "loop", "blocks_per_request"}."""
+    """Sets the library's table from a list of {"kind", "T", "loop", "blocks_per_request"} (and
+    "waves" for the all-ones and repeated-row classes; absent = 0, the built-in choice)."""
     L = lib or _native.load()
     for e in entries:
         rc = L.wh_tune_set(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e["blocks_per_request"]))
+        if rc == 0 and int(e["kind"]) != 1:
+            rc = L.wh_tune_set_waves(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e.get("waves", 0)))
         if rc != 0:
             raise ValueError(L.wh_last_error().decode())
 
@@ -63,6 +71,12 @@ def current(lib=None):
             for loop in (0, 1):
                 out[(kind, T, loop)] = int(L.wh_tune_get(kind, T, loop))
     return out
+
+
+def current_waves(lib=None):
+    """The library's waves table: {(kind, T, loop): waves}, 0 = built-in choice."""
+    L = lib or _native.load()
+    return {k: int(L.wh_tune_get_waves(*k)) for k in current(L) if k[0] != 1}
 
 
 def load(path=None, lib=None):
@@ -142,6 +156,15 @@ def choose(kind, times, margin=0.01):
     return best
 
 
+def choose_waves(times, margin=0.01):
+    """The waves count to keep: the fastest, if it beats the built-in choice (0) by more than
+    ``margin``, else 0."""
+    best = min(times, key=times.get)
+    if times[best] >= (1.0 - margin) * times[0]:
+        return 0
+    return best
+
+
 def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234, log=print):
     """Times every candidate per kernel class on the current device, applies the choice
     (``choose``) and writes the table to ``path`` (None: do not write).  Returns the entries."""
@@ -164,9 +187,19 @@ def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234
         _native.check(L.wh_tune_set(kind, T, loop, int(best)))
         e = {"kind": kind, "T": T, "loop": loop, "blocks_per_request": int(best),
              "ms": {str(c): round(1e3 * v, 4) for c, v in times.items()}}
+        if kind != 1:
+            wtimes = {}
+            for w in (0, 1, 2, 3, 4):
+                _native.check(L.wh_tune_set_waves(kind, T, loop, w))
+                wtimes[w] = _time(preps, reps, torch, stream)
+            wbest = choose_waves(wtimes)
+            _native.check(L.wh_tune_set_waves(kind, T, loop, int(wbest)))
+            e["waves"] = int(wbest)
+            e["waves_ms"] = {str(w): round(1e3 * v, 4) for w, v in wtimes.items()}
         entries.append(e)
         if log:
-            log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s" % (kind, T, loop, best, e["ms"]))
+            log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s, waves %s %s"
+                % (kind, T, loop, best, e["ms"], e.get("waves", "-"), e.get("waves_ms", "")))
     if path is not None:
         doc = {"device": torch.cuda.get_device_name(),
                "candidates": {str(k): list(candidates or v) for k, v in KIND_CANDIDATES.items()}, "entries": entries}
