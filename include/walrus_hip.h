@@ -328,6 +328,15 @@ int wh_tune_set_waves(int kind, int T, int loop, int waves);
 /* The waves entry, or -1 for an entry out of range. */
 int wh_tune_get_waves(int kind, int T, int loop);
 
+/* Tile side for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1) kernel class:
+ * 0 for the class's own side T, or a larger side (at most 8 for kind 0, 2 for kind 2) that the
+ * class's requests then run at (more zero padding), where the class's kernel takes it (else the
+ * entry is ignored); a request's own WH_OPT_TILE takes precedence.  Set by the autotuner. */
+int wh_tune_set_tile(int kind, int T, int loop, int tile);
+
+/* The tile entry, or -1 for an entry out of range. */
+int wh_tune_get_tile(int kind, int T, int loop);
+
 /* How plans and the serving engine launch a batch's complex all-ones requests with n in [26, 48]
  * (the symmetric matrix-core kernel's tile sides 4..6): 0 (default) = one launch per kernel
  * class, then the fold; 1 = one persistent launch for all of them (workgroups take the classes'

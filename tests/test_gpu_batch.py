@@ -397,6 +397,35 @@ def test_forced_tile_sides_and_waves(loop):
         db.close()
 
 
+def test_tuned_tile_side_equals_the_tile_option():
+    """A tile side set in the tuning table (wh_tune_set_tile) runs its class's requests exactly as
+    the request option WH_OPT_TILE does (bit-identical), and a request's own option wins."""
+    from thewalrus_amd import _native
+
+    L = _native.load()
+    rng = np.random.default_rng(95)
+    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), lp) for n in (18, 22, 24) for lp in (False, True)]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]  # all in the tile side 3 class
+
+    def run(**kw):
+        db = B.DeviceBatch(preps, **kw)
+        db.run()
+        r = db.results()
+        db.close()
+        return r
+
+    at4, at5 = run(tile=4), run(tile=5)
+    before = {lp: L.wh_tune_get_tile(0, 3, lp) for lp in (0, 1)}
+    try:
+        for lp in (0, 1):
+            assert L.wh_tune_set_tile(0, 3, lp, 4) == 0
+        assert np.array_equal(run(), at4)
+        assert np.array_equal(run(tile=5), at5)
+    finally:
+        for lp, t in before.items():
+            L.wh_tune_set_tile(0, 3, lp, t)
+
+
 def test_tile_and_waves_that_do_not_fit_are_refused():
     rng = np.random.default_rng(93)
     A = random_symmetric(rng, 20)

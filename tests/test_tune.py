@@ -70,6 +70,8 @@ def test_committed_table_is_valid():
         assert e["blocks_per_request"] == tune.choose(e["kind"], {int(k): v for k, v in e["ms"].items()})
         if "waves_ms" in e:
             assert e["waves"] == tune.choose_waves({int(k): v for k, v in e["waves_ms"].items()})
+        if "tile_ms" in e:
+            assert e["tile"] == tune.choose_waves({int(k): v for k, v in e["tile_ms"].items()})
 
 
 def test_waves_round_trip_and_range(restore_table):
@@ -89,3 +91,20 @@ def test_waves_round_trip_and_range(restore_table):
 def test_choose_waves_keeps_default_within_noise():
     assert tune.choose_waves({0: 10.0, 1: 9.95, 2: 12.0}) == 0
     assert tune.choose_waves({0: 10.0, 2: 9.5}) == 2
+
+
+def test_tile_round_trip_and_range(restore_table):
+    L = _native.load()
+    before = tune.current_tiles()
+    try:
+        assert L.wh_tune_set_tile(0, 3, 1, 4) == 0 and L.wh_tune_get_tile(0, 3, 1) == 4
+        assert L.wh_tune_set_tile(0, 5, 0, 8) == 0 and L.wh_tune_get_tile(0, 5, 0) == 8
+        assert L.wh_tune_set_tile(2, 1, 0, 2) == 0 and L.wh_tune_get_tile(2, 1, 0) == 2
+        assert L.wh_tune_set_tile(0, 3, 1, 0) == 0 and L.wh_tune_get_tile(0, 3, 1) == 0
+        # a side not above the class's own, past the kernels' largest, a general-kernel class
+        for args in ((0, 3, 0, 3), (0, 3, 0, 2), (0, 5, 0, 9), (2, 1, 0, 3), (2, 2, 0, 3), (1, 0, 0, 2)):
+            assert L.wh_tune_set_tile(*args) != 0
+        assert L.wh_tune_get_tile(1, 0, 0) == -1
+    finally:
+        for (k, T, lp), t in before.items():
+            L.wh_tune_set_tile(k, T, lp, t)

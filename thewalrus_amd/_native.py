@@ -73,6 +73,8 @@ EXPORTS = [
     "wh_tune_get",
     "wh_tune_set_waves",
     "wh_tune_get_waves",
+    "wh_tune_set_tile",
+    "wh_tune_get_tile",
     "wh_tune_persistent",
     "wh_loop_hafnian_batch",
     "wh_loop_hafnian_batch_gamma",
@@ -130,7 +132,9 @@ def load(path=None):
         L.wh_tune_get.restype = ctypes.c_int64
         L.wh_tune_persistent.argtypes = [ctypes.c_int]
         L.wh_tune_set_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.wh_tune_set_tile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_tune_get_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.wh_tune_get_tile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_void_p)]
         L.wh_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _ip, _ip, _dp]

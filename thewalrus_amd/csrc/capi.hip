@@ -269,19 +269,28 @@ size_t group_lds_bytes(const GroupKey& g, int nw) {
 
 // The kernel class of a request, with the WH_OPT_TILE / WH_OPT_WAVES overrides applied; fails
 // (kind < 0) when an override does not fit the class.
+int tuned_tile(const GroupKey& g);
+
+bool tile_fits(const GroupKey& g, int tile) {
+    return all_ones_kind(g.kind) ? tile >= g.T && tile <= (g.kind == KIND_PM1S ? kPm1sMaxT : kPm1MaxT) &&
+                                       !(g.kind == KIND_PM1M && tile > 5)
+           : g.kind == KIND_REP ? tile >= g.T && tile <= kRepMaxT
+                                : false;
+}
+
 GroupKey classify(const wh_job_dev& j, int options) {
     GroupKey g = classify_auto(j, options);
     const int tile = (options >> WH_OPT_TILE_SHIFT) & 15;
     const int waves = (options >> WH_OPT_WAVES_SHIFT) & 7;
     if (tile) {
-        const bool ok = all_ones_kind(g.kind) ? tile >= g.T && tile <= (g.kind == KIND_PM1S ? kPm1sMaxT : kPm1MaxT) &&
-                                                    !(g.kind == KIND_PM1M && tile > 5)
-                        : g.kind == KIND_REP ? tile >= g.T && tile <= kRepMaxT : false;
-        if (!ok) {
+        if (!tile_fits(g, tile)) {
             g.kind = -1;
             return g;
         }
         g.T = tile;
+    } else {  // the tuned tile side of the class, where one is set and the class's kernel takes it
+        const int tt = tuned_tile(g);
+        if (tt > g.T && tile_fits(g, tt)) g.T = tt;
     }
     if (waves) {
         if (!(all_ones_kind(g.kind) || g.kind == KIND_REP) || waves > 4 || group_lds_bytes(g, waves) > 160 * 1024) {
@@ -377,6 +386,8 @@ int pm1_waves(int T, bool real, int kind) {
 std::atomic<int64_t> g_tune_blocks[3][kPm1MaxT + 1][2];
 // Waves per workgroup per kernel class (wh_tune_set_waves; 0 = the LDS-based choice), same indexing.
 std::atomic<int> g_tune_waves[3][kPm1MaxT + 1][2];
+// Tile side per kernel class (wh_tune_set_tile; 0 = the class's own side T), same indexing.
+std::atomic<int> g_tune_tile[3][kPm1MaxT + 1][2];
 
 int64_t tuned_blocks(const GroupKey& g) {
     const int kind = all_ones_kind(g.kind) ? 0 : g.kind == KIND_REP ? 2 : 1;
@@ -389,6 +400,11 @@ int tuned_waves(const GroupKey& g) {
     const int kind = all_ones_kind(g.kind) ? 0 : 2;
     const int T = std::min(std::max(g.T, 0), kPm1MaxT);
     return g_tune_waves[kind][T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
+}
+
+int tuned_tile(const GroupKey& g) {
+    if (!(all_ones_kind(g.kind) || g.kind == KIND_REP) || g.T < 1 || g.T > kPm1MaxT) return 0;
+    return g_tune_tile[all_ones_kind(g.kind) ? 0 : 2][g.T][g.loop ? 1 : 0].load(std::memory_order_relaxed);
 }
 
 int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
@@ -884,6 +900,21 @@ int wh_tune_get_waves(int kind, int T, int loop) {
         (kind == 2 && (T < 1 || T > kRepMaxT)))
         return -1;
     return g_tune_waves[kind][T][loop].load(std::memory_order_relaxed);
+}
+
+int wh_tune_set_tile(int kind, int T, int loop, int tile) {
+    if (kind < 0 || kind > 2 || kind == 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
+        (kind == 2 && (T < 1 || T > kRepMaxT)) || (tile != 0 && (tile <= T || tile > (kind == 0 ? kPm1sMaxT : kRepMaxT))))
+        return fail(WH_EINVAL, "tile entry out of range (kind %d, T %d, loop %d, tile %d)", kind, T, loop, tile);
+    g_tune_tile[kind][T][loop].store(tile, std::memory_order_relaxed);
+    return WH_OK;
+}
+
+int wh_tune_get_tile(int kind, int T, int loop) {
+    if (kind < 0 || kind > 2 || kind == 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
+        (kind == 2 && (T < 1 || T > kRepMaxT)))
+        return -1;
+    return g_tune_tile[kind][T][loop].load(std::memory_order_relaxed);
 }
 
 int wh_tune_persistent(int mode) {

@@ -74,6 +74,13 @@ __device__ __forceinline__ double sflip(double v, uint32_t bit) {
 __device__ __forceinline__ cplx cflip(cplx v, uint32_t bit) { return cmk(sflip(v.re, bit), sflip(v.im, bit)); }
 // the partner of index c in the other half (identity on the padding)
 __device__ __forceinline__ int swp(int c, int h) { return c < h ? c + h : (c < 2 * h ? c - h : c); }
+// an LDS value; the real instance (RE) reads the real half only, so the compiler sees a zero
+// imaginary part and drops the arithmetic on it (imaginary halves in LDS are then never read)
+template <bool RE>
+__device__ __forceinline__ cplx ldc(const cplx* p) {
+    if constexpr (RE) return cmk(p->re, 0.0);
+    else return *p;
+}
 
 template <int T> struct GeoS {
     static constexpr int NP = 8 * T;
@@ -93,13 +100,13 @@ struct LaneMapS {
 };
 
 // [eta^h] exp(sum_i a_i eta^i) by the exponential recurrence (as sieve_pm1m.hip)
-template <int H, bool LOOP>
+template <int H, bool LOOP, bool RE = false>
 __device__ __forceinline__ cplx coeff_top_s(const cplx* pt, const cplx* lv, cplx v1, int h) {
     cplx b[H + 1], e[H + 1];
 #pragma unroll
     for (int i = 1; i <= H; ++i) {
-        cplx v = cmulr(pt[i], 0.5);
-        if (LOOP) v = cadd(v, cmulr(i == 1 ? v1 : lv[i], 0.5 * i));
+        cplx v = cmulr(ldc<RE>(pt + i), 0.5);
+        if (LOOP) v = cadd(v, cmulr(i == 1 ? v1 : ldc<RE>(lv + i), 0.5 * i));
         b[i] = i <= h ? v : cmk(0.0, 0.0);
     }
     e[0] = cmk(1.0, 0.0);
@@ -125,7 +132,7 @@ template <int T>
 struct TdotVals {
     cplx x[T * (T + 1) / 2];
 };
-template <int T>
+template <int T, bool RE = false>
 __device__ __forceinline__ void sym_tdot_load(TdotVals<T>& X, const cplx* Vb, const LaneMapS& L, int h) {
     constexpr int S = GeoS<T>::S;
     int g = 0;
@@ -133,7 +140,7 @@ __device__ __forceinline__ void sym_tdot_load(TdotVals<T>& X, const cplx* Vb, co
     for (int p = 0; p < T; ++p) {
         const int rs = swp(L.rd(p), h);
 #pragma unroll
-        for (int q = p; q < T; ++q) X.x[g++] = Vb[swp(L.cd(q), h) * S + rs];
+        for (int q = p; q < T; ++q) X.x[g++] = ldc<RE>(&Vb[swp(L.cd(q), h) * S + rs]);
     }
 }
 template <int T>
@@ -157,7 +164,7 @@ __device__ __forceinline__ cplx sym_tdot_sum(const TdotVals<T>& X, const double 
 
 // V = R diag(d) into LDS: each upper group at its place and, off the diagonal, at the mirrored
 // place (V is symmetric); column sign = bit T + q of the lane's sign mask
-template <int T>
+template <int T, bool RE = false>
 __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], const double (&aI)[T][T],
                                           const LaneMapS& L, uint32_t fm) {
     constexpr int S = GeoS<T>::S;
@@ -166,8 +173,13 @@ __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], con
 #pragma unroll
         for (int q = p; q < T; ++q) {
             const cplx v = cflip(cmk(aR[p][q], aI[p][q]), (fm >> (T + q)) & 1u);
-            X[L.rd(p) * S + L.cd(q)] = v;
-            if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
+            if constexpr (RE) {  // the real halves only
+                X[L.rd(p) * S + L.cd(q)].re = v.re;
+                if (p < q) X[L.cd(q) * S + L.rd(p)].re = v.re;
+            } else {
+                X[L.rd(p) * S + L.cd(q)] = v;
+                if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
+            }
         }
 }
 
@@ -193,31 +205,31 @@ __device__ __forceinline__ int epis_steps(int kc) {
     return n;
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void epis_init(EpiS<T>& E, const LaneMapS& L, const cplx* zprev, uint32_t fm,
                                           int h) {
     E.qe = E.qf = E.qt = E.de = E.dodd = E.zs = cmk(0.0, 0.0);
     if (LOOP) {  // t = X D_d z_(a-1) at the lane's columns
 #pragma unroll
-        for (int q = 0; q < T; ++q) E.tv[q] = cflip(zprev[swp(L.cd(q), h)], (fm >> (T + q)) & 1u);
+        for (int q = 0; q < T; ++q) E.tv[q] = cflip(ldc<RE>(&zprev[swp(L.cd(q), h)]), (fm >> (T + q)) & 1u);
     }
 }
 
 // the operands of row group r's micro-steps
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS& L, const cplx* Vb,
                                               const cplx* Ms, int h) {
     constexpr int S = GeoS<T>::S;
     const int rs = swp(L.rd(r), h);
 #pragma unroll
-    for (int q = r; q < T; ++q) E.tx[q] = Vb[swp(L.cd(q), h) * S + rs];
+    for (int q = r; q < T; ++q) E.tx[q] = ldc<RE>(&Vb[swp(L.cd(q), h) * S + rs]);
     if (LOOP) {  // A[rd(r)][cd(q)] = (X A)[swap rd(r)][cd(q)]
 #pragma unroll
-        for (int q = 0; q < T; ++q) E.av[q] = Ms[rs * S + L.cd(q)];
+        for (int q = 0; q < T; ++q) E.av[q] = ldc<RE>(&Ms[rs * S + L.cd(q)]);
     }
 }
 
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const double (&oR)[T][T],
                                           const double (&oI)[T][T], const LaneMapS& L, uint32_t fm, int lane,
                                           int a_prev, cplx qo_prev, const cplx* zprev, cplx* zcur, cplx* pt,
@@ -256,9 +268,9 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
         const int e = lane < n ? lane : 0;
         if (st == 0) {
             wave_lds_sync();  // the z rows of every lane are stored
-            E.zc[0] = zcur[e];
-            E.zc[1] = zprev[e];
-            E.zc[2] = zcur[swp(e, h)];
+            E.zc[0] = ldc<RE>(&zcur[e]);
+            E.zc[1] = ldc<RE>(&zprev[e]);
+            E.zc[2] = ldc<RE>(&zcur[swp(e, h)]);
             return;
         }
         if (st == 1) {
@@ -269,7 +281,9 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
         }
         st -= 2;
     }
-    const double v[8] = {qo_prev.re, qo_prev.im, E.qt.re, E.qt.im, E.de.re, E.de.im, E.dodd.re, E.dodd.im};
+    // (the real instance's imaginary parts are zeros of either sign: zero them for the compiler)
+    const double v[8] = {qo_prev.re, RE ? 0.0 : qo_prev.im, E.qt.re,   RE ? 0.0 : E.qt.im,
+                         E.de.re,    RE ? 0.0 : E.de.im,    E.dodd.re, RE ? 0.0 : E.dodd.im};
     if (st < 4) { E.s1[st] = dswap32(v[st], v[st + 4]); return; }
     if (st < 6) { E.s2[st - 4] = dswap16(E.s1[st - 4], E.s1[st - 2]); return; }
     if (st == 6) {
@@ -331,7 +345,7 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
             if (G3) nY[p][q] = 0.0;
         }
     EpiS<T> E;
-    epis_init<T, LOOP>(E, L, zprev, fm, h);
+    epis_init<T, LOOP, RE>(E, L, zprev, fm, h);
     const cplx* pa = Vb + (4 * L.br + L.x) * S + L.y;
     const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
     cplx avn[T], bqn[T];
@@ -339,7 +353,7 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
     for (int p = 0; p < T; ++p) avn[p] = RE ? cmk(pa[8 * p * S].re, 0.0) : pa[8 * p * S];
 #pragma unroll
     for (int q = 0; q < T; ++q) bqn[q] = RE ? cmk(pb[8 * q].re, 0.0) : pb[8 * q];
-    epis_load_row<T, LOOP>(0, E, L, Vb, Ms, h);
+    epis_load_row<T, LOOP, RE>(0, E, L, Vb, Ms, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
         cplx av[T], bq[T];
@@ -355,7 +369,7 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
             for (int q = 0; q < T; ++q)
                 bqn[q] = RE ? cmk(pb[4 * (kc + 1) * S + 8 * q].re, 0.0) : pb[4 * (kc + 1) * S + 8 * q];
         }
-        if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP>((kc + 1) >> 1, E, L, Vb, Ms, h);
+        if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP, RE>((kc + 1) >> 1, E, L, Vb, Ms, h);
         double as[T], bs[T];
         if (G3) {
 #pragma unroll
@@ -372,7 +386,7 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
                 nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
                 if (G3) nY[p][q] = mfma4(av[p].im, bq[q].im, nY[p][q]);
                 else if (!RE) nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
-                if (st < nst) epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                if (st < nst) epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                 ++st;
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -388,13 +402,13 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
                         nI[p][q] = mfma4(av[p].im, bq[q].re, nI[p][q]);
                     }
                     if (st < nst)
-                        epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+                        epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                     ++st;
                     __builtin_amdgcn_sched_barrier(0);
                 }
         }
         for (; st < nst; ++st)
-            epis_step<T, LOOP>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
+            epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
     }
     if (G3) {
 #pragma unroll
@@ -409,21 +423,21 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
 }
 
 // the last power's epilogue, the same micro-steps in sequence
-template <int T, bool LOOP>
+template <int T, bool LOOP, bool RE = false>
 __device__ __forceinline__ void epis_alone(const double (&aR)[T][T], const double (&aI)[T][T], int a, cplx qo,
                                            const LaneMapS& L, uint32_t fm, int lane, const cplx* Vb,
                                            const cplx* Ms, const cplx* zprev, cplx* zcur, uint64_t tu, int h,
                                            cplx* pt, cplx* lv) {
     constexpr int NK = 2 * T;
     EpiS<T> E;
-    epis_init<T, LOOP>(E, L, zprev, fm, h);
+    epis_init<T, LOOP, RE>(E, L, zprev, fm, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
-        if ((kc & 1) == 0) epis_load_row<T, LOOP>(kc >> 1, E, L, Vb, Ms, h);
+        if ((kc & 1) == 0) epis_load_row<T, LOOP, RE>(kc >> 1, E, L, Vb, Ms, h);
         const int nst = epis_steps<T, LOOP>(kc);
 #pragma unroll
         for (int st = 0; st < 2 * T + 18; ++st)
-            if (st < nst) epis_step<T, LOOP>(kc, st, E, aR, aI, L, fm, lane, a, qo, zprev, zcur, pt, lv, tu, h);
+            if (st < nst) epis_step<T, LOOP, RE>(kc, st, E, aR, aI, L, fm, lane, a, qo, zprev, zcur, pt, lv, tu, h);
     }
 }
 
@@ -501,14 +515,14 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = p; q < T; ++q) {
-                const cplx m = cflip(Ms[swp(L.rd(p), h) * S + L.cd(q)], (fm >> p) & 1u);
+                const cplx m = cflip(ldc<RE>(&Ms[swp(L.rd(p), h) * S + L.cd(q)]), (fm >> p) & 1u);
                 aR[p][q] = m.re;
-                aI[p][q] = m.im;
+                aI[p][q] = RE ? 0.0 : m.im;
             }
-        sym_store<T>(Pw, aR, aI, L, fm);
+        sym_store<T, RE>(Pw, aR, aI, L, fm);
         // p_1 = tr(B) = sum_e d_e A[e][swap e] (lane partials, summed with power 1's results)
         cplx qo = cmk(0.0, 0.0);
-        if (lane < n) qo = cmulr(Ms[swp(lane, h) * S + swp(lane, h)], sgn(tu, lane, h));
+        if (lane < n) qo = cmulr(ldc<RE>(&Ms[swp(lane, h) * S + swp(lane, h)]), sgn(tu, lane, h));
         if (LOOP) {
             for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];  // z_0 = D
         }
@@ -525,38 +539,39 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
             for (; a + 1 <= Ahalf; a += 2) {
                 sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T>(tv, Pw, L, h);  // Pw holds V_(a-1): for tr(B^(2a-1))
+                sym_tdot_load<T, RE>(tv, Pw, L, h);  // Pw holds V_(a-1): for tr(B^(2a-1))
                 wave_lds_sync();
-                sym_store<T>(Pw, bR, bI, L, fm);
+                sym_store<T, RE>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
                 qo = sym_tdot_sum<T>(tv, bR, bI, fm);
                 sym_product_epi<T, LOOP, RE>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
                                          Zb + (a & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T>(tv, Pw, L, h);
+                sym_tdot_load<T, RE>(tv, Pw, L, h);
                 wave_lds_sync();
-                sym_store<T>(Pw, aR, aI, L, fm);
+                sym_store<T, RE>(Pw, aR, aI, L, fm);
                 wave_lds_sync();
                 qo = sym_tdot_sum<T>(tv, aR, aI, fm);
             }
             if (a <= Ahalf) {  // one power left: R_a into b, then its epilogue alone
                 sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
                                          Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T>(tv, Pw, L, h);
+                sym_tdot_load<T, RE>(tv, Pw, L, h);
                 wave_lds_sync();
-                sym_store<T>(Pw, bR, bI, L, fm);
+                sym_store<T, RE>(Pw, bR, bI, L, fm);
                 wave_lds_sync();
                 qo = sym_tdot_sum<T>(tv, bR, bI, fm);
-                epis_alone<T, LOOP>(bR, bI, a, qo, L, fm, lane, Pw, Ms, Zb + ((a - 1) & 1) * NP, Zb + (a & 1) * NP,
+                epis_alone<T, LOOP, RE>(bR, bI, a, qo, L, fm, lane, Pw, Ms, Zb + ((a - 1) & 1) * NP, Zb + (a & 1) * NP,
                                     tu, h, pt, lv);
             } else {  // the last power (a - 1) is in a
-                epis_alone<T, LOOP>(aR, aI, a - 1, qo, L, fm, lane, Pw, Ms, Zb + (a & 1) * NP,
+                epis_alone<T, LOOP, RE>(aR, aI, a - 1, qo, L, fm, lane, Pw, Ms, Zb + (a & 1) * NP,
                                     Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
             }
         }
         if (LOOP) {
             // v_1 = XD_d . D = sum_e d_e D[swap e] D[e] (every lane holds the sum)
             cplx part = cmk(0.0, 0.0);
-            for (int e = lane; e < n; e += WH_WAVE) cmac(part, cmulr(Dv[swp(e, h)], sgn(tu, e, h)), Dv[e]);
+            for (int e = lane; e < n; e += WH_WAVE)
+                cmac(part, cmulr(ldc<RE>(&Dv[swp(e, h)]), sgn(tu, e, h)), ldc<RE>(&Dv[e]));
             const cplx v1 = wave_csum(part);
             if (lane == 0) lv[1] = v1;
         }
@@ -568,7 +583,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
                 const cplx* ptj = slots + lane * 2 * PTN;
                 const uint64_t tj = (uint64_t)(t_batch + (int64_t)lane * nw);
                 const double prefac = ((h - __popcll(tj)) & 1) ? -1.0 : 1.0;
-                const cplx top = coeff_top_s<4 * T, LOOP>(ptj, ptj + PTN, ptj[PTN + 1], h);
+                const cplx top = coeff_top_s<4 * T, LOOP, RE>(ptj, ptj + PTN, ldc<RE>(ptj + PTN + 1), h);
                 acc_sum.add(cmulr(top, prefac));
             }
             wave_lds_sync();  // the slots are free again

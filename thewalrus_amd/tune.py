@@ -17,7 +17,9 @@ the built-in default by more than 1% (`choose`):
 For the all-ones and repeated-row classes the tuner then times the waves per workgroup, 1 to 4
 against the built-in choice (the count that fits the most waves per CU in the LDS), and keeps a
 count only when it beats that choice by more than 1% (`wh_tune_set_waves`; a count whose
-workgroup does not fit the LDS is ignored by the library).
+workgroup does not fit the LDS is ignored by the library).  Last it times the next larger tile
+side for those classes (all-ones T = 1..5, repeated-row T = 1: the same requests with more zero
+padding) and keeps it only when it wins by more than 1% (`wh_tune_set_tile`).
 
 The table lives in a JSON file (`tuning.json` next to this module by default, or the path in
 the environment variable ``WH_TUNING``; ``WH_TUNING=none`` keeps the built-in defaults) that the
@@ -58,6 +60,8 @@ def apply(entries, lib=None):
         rc = L.wh_tune_set(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e["blocks_per_request"]))
         if rc == 0 and int(e["kind"]) != 1:
             rc = L.wh_tune_set_waves(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e.get("waves", 0)))
+        if rc == 0 and int(e["kind"]) != 1:
+            rc = L.wh_tune_set_tile(int(e["kind"]), int(e["T"]), int(e["loop"]), int(e.get("tile", 0)))
         if rc != 0:
             raise ValueError(L.wh_last_error().decode())
 
@@ -77,6 +81,12 @@ def current_waves(lib=None):
     """The library's waves table: {(kind, T, loop): waves}, 0 = built-in choice."""
     L = lib or _native.load()
     return {k: int(L.wh_tune_get_waves(*k)) for k in current(L) if k[0] != 1}
+
+
+def current_tiles(lib=None):
+    """The library's tile table: {(kind, T, loop): tile side}, 0 = the class's own side."""
+    L = lib or _native.load()
+    return {k: int(L.wh_tune_get_tile(*k)) for k in current(L) if k[0] != 1}
 
 
 def load(path=None, lib=None):
@@ -157,8 +167,8 @@ def choose(kind, times, margin=0.01):
 
 
 def choose_waves(times, margin=0.01):
-    """The waves count to keep: the fastest, if it beats the built-in choice (0) by more than
-    ``margin``, else 0."""
+    """The waves count (or tile side) to keep: the fastest, if it beats the built-in choice (0)
+    by more than ``margin``, else 0."""
     best = min(times, key=times.get)
     if times[best] >= (1.0 - margin) * times[0]:
         return 0
@@ -196,10 +206,20 @@ def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234
             _native.check(L.wh_tune_set_waves(kind, T, loop, int(wbest)))
             e["waves"] = int(wbest)
             e["waves_ms"] = {str(w): round(1e3 * v, 4) for w, v in wtimes.items()}
+            if (kind == 0 and T <= 5) or (kind == 2 and T == 1):
+                ttimes = {}
+                for tl in (0, T + 1):
+                    _native.check(L.wh_tune_set_tile(kind, T, loop, tl))
+                    ttimes[tl] = _time(preps, reps, torch, stream)
+                tbest = choose_waves(ttimes)
+                _native.check(L.wh_tune_set_tile(kind, T, loop, int(tbest)))
+                e["tile"] = int(tbest)
+                e["tile_ms"] = {str(tl): round(1e3 * v, 4) for tl, v in ttimes.items()}
         entries.append(e)
         if log:
-            log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s, waves %s %s"
-                % (kind, T, loop, best, e["ms"], e.get("waves", "-"), e.get("waves_ms", "")))
+            log("class kind=%d T=%d loop=%d: keep %d workgroups/request %s, waves %s %s, tile %s %s"
+                % (kind, T, loop, best, e["ms"], e.get("waves", "-"), e.get("waves_ms", ""), e.get("tile", "-"),
+                   e.get("tile_ms", "")))
     if path is not None:
         doc = {"device": torch.cuda.get_device_name(),
                "candidates": {str(k): list(candidates or v) for k, v in KIND_CANDIDATES.items()}, "entries": entries}
