@@ -326,15 +326,82 @@ constexpr bool pm1s_3m() {
     return !RE && WH_PM1S_3M && (T <= 5 || (T == 6 && !LOOP));
 }
 
+// Product boundary inside the last k chunk (WH_PM1S_FUSED, on by default for the instances
+// without register spills): the trace loads of V_(a_prev) issue at the start of the last chunk
+// (into the registers the operand prefetch no longer needs), and each group's final value
+// (the three-product combination where it applies), its share of tr(B^(2 a_prev + 1)) and its
+// V stores follow two groups behind the group's last MFMA, in the matrix cores' shadow.  The
+// plain form runs them after the product: 13% of the T = 5 loop launch; fused, n = 32..40 run
+// 3.5-4.6% faster and the real n = 24 batch 3.5%.  The complex T = 6 and real T = 5 instances
+// keep the plain form (with the fused one they spill or lose the full unroll of the product),
+// and so does the complex T = 3 loop instance (n = 24 loop: 1.44 -> 1.67 ms fused, with the
+// same instruction counts: its short last chunk leaves the combination waiting on the MFMAs).
+#ifndef WH_PM1S_FUSED
+#define WH_PM1S_FUSED 1
+#endif
+template <int T, bool LOOP, bool RE>
+constexpr bool pm1s_fused() {
+    return WH_PM1S_FUSED && (RE ? T <= 4 : T <= 5 && !(T == 3 && LOOP));
+}
+
+// Fused boundary state: the trace loads and the running trace sums of the last chunk.
+template <int T>
+struct BndS {
+    TdotVals<T> tv;
+    cplx sd, so, s;
+};
+
+// group gi (upper groups in row-major order) of the fused boundary: the final value (Gauss's
+// combination), the group's term of tr(B^(2 a_prev + 1)) and the stores of V = R diag(d)
+template <int T, bool G3, bool RE>
+__device__ __forceinline__ void bnd_step(int gi, BndS<T>& Bd, cplx* X, double (&nR)[T][T], double (&nI)[T][T],
+                                         double (&nY)[T][T], const LaneMapS& L, uint32_t fm) {
+    constexpr int S = GeoS<T>::S;
+    int p = 0, q = 0, g = 0;
+#pragma unroll
+    for (int pp = 0; pp < T; ++pp)
+#pragma unroll
+        for (int qq = pp; qq < T; ++qq, ++g)
+            if (g == gi) { p = pp; q = qq; }
+    if (G3) {
+        const double x = nR[p][q], y = nY[p][q];
+        nR[p][q] = x - y;
+        nI[p][q] = nI[p][q] - (x + y);
+    }
+    const cplx r = cmk(nR[p][q], RE ? 0.0 : nI[p][q]);
+    if (q == p) cmac(Bd.sd, r, Bd.tv.x[gi]);
+    else cmac(Bd.so, r, Bd.tv.x[gi]);
+    if (q == T - 1) {  // the row group's partial sum, times its row sign
+        const cplx part = cflip(cadd(Bd.sd, cmulr(Bd.so, 2.0)), (fm >> p) & 1u);
+        Bd.s = p == 0 ? part : cadd(Bd.s, part);
+        Bd.sd = Bd.so = cmk(0.0, 0.0);
+    }
+    const cplx v = cflip(r, (fm >> (T + q)) & 1u);
+    if constexpr (RE) {
+        X[L.rd(p) * S + L.cd(q)].re = v.re;
+        if (p < q) X[L.cd(q) * S + L.rd(p)].re = v.re;
+    } else {
+        X[L.rd(p) * S + L.cd(q)] = v;
+        if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
+    }
+}
+
+// With the fused boundary (pm1s_fused) the product also overwrites Vb with V_a and returns the
+// lane's part of tr(B^(2 a_prev + 1)); otherwise it returns zero and the caller does both.
 template <int T, bool LOOP, bool RE = false>
-__device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, const cplx* __restrict__ Ms,
+__device__ __forceinline__ cplx sym_product_epi(cplx* Vb, const cplx* __restrict__ Ms,
                                                 double (&nR)[T][T], double (&nI)[T][T], const LaneMapS& L,
                                                 uint32_t fm, const double (&oR)[T][T], const double (&oI)[T][T],
                                                 int a_prev, cplx qo_prev, int lane, const cplx* zprev,
                                                 cplx* zcur, uint64_t tu, int h, cplx* pt, cplx* lv) {
     constexpr int S = GeoS<T>::S;
     constexpr int NK = 2 * T;
+    constexpr int NG = T * (T + 1) / 2;
     constexpr bool G3 = pm1s_3m<T, LOOP, RE>();
+    constexpr bool FB = pm1s_fused<T, LOOP, RE>();
+    constexpr int LAG = 2;  // groups between a group's last MFMA and its boundary step
+    BndS<T> Bd;
+    Bd.sd = Bd.so = Bd.s = cmk(0.0, 0.0);
     double nY[T][T];  // G3: sum of Im a Im b
 #pragma unroll
     for (int p = 0; p < T; ++p)
@@ -370,6 +437,12 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
                 bqn[q] = RE ? cmk(pb[4 * (kc + 1) * S + 8 * q].re, 0.0) : pb[4 * (kc + 1) * S + 8 * q];
         }
         if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP, RE>((kc + 1) >> 1, E, L, Vb, Ms, h);
+        if (FB && kc == NK - 1) {
+            // V_(a_prev) for the trace, before any store of V_a (one wave's LDS accesses run in
+            // issue order; the fence keeps the compiler from moving the stores above the loads)
+            sym_tdot_load<T, RE>(Bd.tv, Vb, L, h);
+            wave_lds_sync();
+        }
         double as[T], bs[T];
         if (G3) {
 #pragma unroll
@@ -379,22 +452,25 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
         }
         const int nst = epis_steps<T, LOOP>(kc);
         int st = 0;
+        int gl = 0;  // group index of the MFMA loop
 #pragma unroll
         for (int p = 0; p < T; ++p)
 #pragma unroll
-            for (int q = p; q < T; ++q) {
+            for (int q = p; q < T; ++q, ++gl) {
                 nR[p][q] = mfma4(av[p].re, bq[q].re, nR[p][q]);
                 if (G3) nY[p][q] = mfma4(av[p].im, bq[q].im, nY[p][q]);
                 else if (!RE) nI[p][q] = mfma4(av[p].re, bq[q].im, nI[p][q]);
                 if (st < nst) epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                 ++st;
+                if (FB && RE && kc == NK - 1 && gl >= LAG) bnd_step<T, G3, RE>(gl - LAG, Bd, Vb, nR, nI, nY, L, fm);
                 __builtin_amdgcn_sched_barrier(0);
             }
         if (!RE) {
+            int gz = 0;
 #pragma unroll
             for (int p = 0; p < T; ++p)
 #pragma unroll
-                for (int q = p; q < T; ++q) {
+                for (int q = p; q < T; ++q, ++gz) {
                     if (G3) {
                         nI[p][q] = mfma4(as[p], bs[q], nI[p][q]);
                     } else {
@@ -404,21 +480,55 @@ __device__ __forceinline__ void sym_product_epi(const cplx* __restrict__ Vb, con
                     if (st < nst)
                         epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
                     ++st;
+                    if (FB && kc == NK - 1 && gz >= LAG) bnd_step<T, G3, RE>(gz - LAG, Bd, Vb, nR, nI, nY, L, fm);
                     __builtin_amdgcn_sched_barrier(0);
                 }
         }
         for (; st < nst; ++st)
             epis_step<T, LOOP, RE>(kc, st, E, oR, oI, L, fm, lane, a_prev, qo_prev, zprev, zcur, pt, lv, tu, h);
     }
-    if (G3) {
+    if constexpr (FB) {
 #pragma unroll
-        for (int p = 0; p < T; ++p)
+        for (int g = NG - LAG; g < NG; ++g)
+            if (g >= 0) bnd_step<T, G3, RE>(g, Bd, Vb, nR, nI, nY, L, fm);
+        return Bd.s;
+    } else {
+        if (G3) {
 #pragma unroll
-            for (int q = p; q < T; ++q) {
-                const double x = nR[p][q], y = nY[p][q];
-                nR[p][q] = x - y;
-                nI[p][q] = nI[p][q] - (x + y);
-            }
+            for (int p = 0; p < T; ++p)
+#pragma unroll
+                for (int q = p; q < T; ++q) {
+                    const double x = nR[p][q], y = nY[p][q];
+                    nR[p][q] = x - y;
+                    nI[p][q] = nI[p][q] - (x + y);
+                }
+        }
+        return cmk(0.0, 0.0);
+    }
+}
+
+// One product of the power chain and its boundary: R_a = V_(a_prev) (X A) into (nR, nI) with
+// power a_prev's epilogue, V_a into Pw, and the lane's part of tr(B^(2 a_prev + 1)) returned.
+template <int T, bool LOOP, bool RE>
+__device__ __forceinline__ cplx product_step(cplx* Pw, const cplx* __restrict__ Ms, double (&nR)[T][T],
+                                             double (&nI)[T][T], const LaneMapS& L, uint32_t fm,
+                                             const double (&oR)[T][T], const double (&oI)[T][T], int a_prev,
+                                             cplx qo_prev, int lane, const cplx* zprev, cplx* zcur, uint64_t tu,
+                                             int h, cplx* pt, cplx* lv) {
+    if constexpr (pm1s_fused<T, LOOP, RE>()) {
+        const cplx qo = sym_product_epi<T, LOOP, RE>(Pw, Ms, nR, nI, L, fm, oR, oI, a_prev, qo_prev, lane, zprev,
+                                                     zcur, tu, h, pt, lv);
+        wave_lds_sync();  // V_a of every lane is stored
+        return qo;
+    } else {
+        sym_product_epi<T, LOOP, RE>(Pw, Ms, nR, nI, L, fm, oR, oI, a_prev, qo_prev, lane, zprev, zcur, tu, h, pt,
+                                     lv);
+        TdotVals<T> tv;
+        sym_tdot_load<T, RE>(tv, Pw, L, h);  // Pw holds V_(a_prev): for tr(B^(2 a_prev + 1))
+        wave_lds_sync();
+        sym_store<T, RE>(Pw, nR, nI, L, fm);
+        wave_lds_sync();
+        return sym_tdot_sum<T>(tv, nR, nI, fm);
     }
 }
 
@@ -533,33 +643,17 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
         const int Ahalf = (h + 1) / 2;
         {
             double bR[T][T], bI[T][T];
-            TdotVals<T> tv;
             int a = 2;
 #pragma unroll 1
             for (; a + 1 <= Ahalf; a += 2) {
-                sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
-                                         Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T, RE>(tv, Pw, L, h);  // Pw holds V_(a-1): for tr(B^(2a-1))
-                wave_lds_sync();
-                sym_store<T, RE>(Pw, bR, bI, L, fm);
-                wave_lds_sync();
-                qo = sym_tdot_sum<T>(tv, bR, bI, fm);
-                sym_product_epi<T, LOOP, RE>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane, Zb + ((a + 1) & 1) * NP,
-                                         Zb + (a & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T, RE>(tv, Pw, L, h);
-                wave_lds_sync();
-                sym_store<T, RE>(Pw, aR, aI, L, fm);
-                wave_lds_sync();
-                qo = sym_tdot_sum<T>(tv, aR, aI, fm);
+                qo = product_step<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                                               Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
+                qo = product_step<T, LOOP, RE>(Pw, Ms, aR, aI, L, fm, bR, bI, a, qo, lane,
+                                               Zb + ((a + 1) & 1) * NP, Zb + (a & 1) * NP, tu, h, pt, lv);
             }
             if (a <= Ahalf) {  // one power left: R_a into b, then its epilogue alone
-                sym_product_epi<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
-                                         Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
-                sym_tdot_load<T, RE>(tv, Pw, L, h);
-                wave_lds_sync();
-                sym_store<T, RE>(Pw, bR, bI, L, fm);
-                wave_lds_sync();
-                qo = sym_tdot_sum<T>(tv, bR, bI, fm);
+                qo = product_step<T, LOOP, RE>(Pw, Ms, bR, bI, L, fm, aR, aI, a - 1, qo, lane, Zb + (a & 1) * NP,
+                                               Zb + ((a - 1) & 1) * NP, tu, h, pt, lv);
                 epis_alone<T, LOOP, RE>(bR, bI, a, qo, L, fm, lane, Pw, Ms, Zb + ((a - 1) & 1) * NP, Zb + (a & 1) * NP,
                                     tu, h, pt, lv);
             } else {  // the last power (a - 1) is in a
