@@ -450,7 +450,8 @@ def dominant_kernel(step0, B, torch, stream):
 
 def kernel_probe(what, B, torch, stream, reps=5):
     """Launch sequences for kernel traces: the repeated-row batch of configuration 4 on the
-    default route and on the general kernel (config4), or the first 1000 requests of the stream as
+    default route (tile sides 3 and 1), with every request at tile side 2 (2E <= 16, one term per
+    wave) and on the general kernel (config4), or the first 1000 requests of the stream as
     batches of 500 with wh_tune_persistent 0, 1 and 2 (persistent); wall seconds per pass."""
     from thewalrus_amd import _native
 
@@ -458,7 +459,7 @@ def kernel_probe(what, B, torch, stream, reps=5):
     if what == "config4":
         preps = [B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in W.config4()]
         vals = {}
-        for name, kw in (("default", {}), ("general_kernel", {"force_generic": True})):
+        for name, kw in (("default", {}), ("tile_side_2", {"tile": 2}), ("general_kernel", {"force_generic": True})):
             b = B.DeviceBatch(preps, **kw)
             b.run(stream)
             ev, wall = timed_runs(b, reps, torch, stream)
@@ -466,6 +467,7 @@ def kernel_probe(what, B, torch, stream, reps=5):
             vals[name] = b.results()
             b.close()
         out["max_rel_diff"] = max_rel(vals["default"], vals["general_kernel"])
+        out["max_rel_diff_side_2"] = max_rel(vals["default"], vals["tile_side_2"])
         return out
     L = _native.load()
     prev = L.wh_tune_persistent(-1)

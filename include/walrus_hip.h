@@ -128,8 +128,9 @@ typedef struct wh_request {
 #define WH_OPT_CHARPOLY 32
 /* wh_request.options: run an all-ones or repeated-row request at tile side t (n_pad = 8t) instead
  * of the smallest side that holds its matrix (more zero padding; t <= 8 on the symmetric
- * matrix-core kernel, 6 on the vector kernel, 5 on the full-power matrix-core kernel, 2 on the
- * repeated-row kernel), and with w waves per workgroup (1..4, within the LDS) instead of the
+ * matrix-core kernel, 6 on the vector kernel, 5 on the full-power matrix-core kernel; the
+ * repeated-row kernel's sides are 1 (2E <= 8), 3 (2E <= 12: four terms per wave, n_pad = 12) and
+ * 2 (2E <= 16)), and with w waves per workgroup (1..4, within the LDS) instead of the
  * count that fits the most waves per CU.  The batch fails with WH_EINVAL for a value the request's
  * kernel cannot take.  For the tuner and for tests of the larger tile classes on small inputs. */
 #define WH_OPT_TILE_SHIFT 8
@@ -310,8 +311,9 @@ int wh_graph_destroy(wh_graph* graph);
 /* ---- Term-chunk tuning.  No counterpart in the reference (numba's prange picks its own
  * chunks); set by the autotuner (thewalrus_amd/tune.py).  The chunk size of a request aims at
  * `blocks_per_request` workgroups (0 = the default: 1024 for the all-ones kernels, 256 for the
- * general kernel), per kernel class: kind 0 = all-ones kernels with tile side class T = 1..6
- * (n_pad = 8T), kind 1 = general kernel with T = 0; loop = 0 or 1.  Plans and batches created
+ * general kernel, 64 for the repeated-row kernel), per kernel class: kind 0 = all-ones kernels
+ * with tile side class T = 1..6 (n_pad = 8T), kind 1 = general kernel with T = 0, kind 2 =
+ * repeated-row kernel with tile side T = 1..3 (as WH_OPT_TILE); loop = 0 or 1.  Plans and batches created
  * afterwards use the table; a request's result depends on its chunking only in the last bits
  * (summation order). */
 int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
@@ -319,7 +321,7 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
 /* The table entry, or -1 for an entry out of range. */
 int64_t wh_tune_get(int kind, int T, int loop);
 
-/* Waves per workgroup for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1, 2)
+/* Waves per workgroup for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1..3)
  * kernel class, 1..4, or 0 for the default (the count that fits the most waves per CU in the
  * LDS); a value whose workgroup would not fit the LDS is ignored.  Set by the autotuner; results
  * change in their last bits at most (the order in which a chunk's terms are summed). */
@@ -328,8 +330,9 @@ int wh_tune_set_waves(int kind, int T, int loop, int waves);
 /* The waves entry, or -1 for an entry out of range. */
 int wh_tune_get_waves(int kind, int T, int loop);
 
-/* Tile side for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1) kernel class:
- * 0 for the class's own side T, or a larger side (at most 8 for kind 0, 2 for kind 2) that the
+/* Tile side for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1..3) kernel class:
+ * 0 for the class's own side T, or a larger side (at most 8 for kind 0; for kind 2 one that
+ * holds a larger 2E: 3 or 2 for T = 1, 2 for T = 3) that the
  * class's requests then run at (more zero padding), where the class's kernel takes it (else the
  * entry is ignored); a request's own WH_OPT_TILE takes precedence.  Set by the autotuner. */
 int wh_tune_set_tile(int kind, int T, int loop, int tile);

@@ -367,6 +367,53 @@ def test_repeated_row_kernel_matches_generic_and_truth(real, glynn):
             assert relerr(g, truth) < bar, (rpt, loop, g, truth)
 
 
+REP_SIDE_CAP = {1: 8, 3: 12, 2: 16}  # the largest 2E each repeated-row tile side holds
+
+
+@pytest.mark.parametrize("loop", [False, True])
+def test_repeated_row_tile_sides_agree_with_truth(loop):
+    """The repeated-row kernel's tile sides -- 1 (2E <= 8, one term per wave on 8 x 8), 3
+    (2E <= 12: four terms per wave, 16 lanes each on 12 x 12) and 2 (2E <= 16) -- on every request
+    each can hold, real and complex, Glynn and inclusion-exclusion, against the double-double
+    oracle; side 3 also with 1..4 waves per workgroup, and on requests of configuration 4's shape
+    (8 modes, 32 photons, loop weights, about 500 terms: many chunks, the last ending inside a
+    wave's group of four terms)."""
+    from thewalrus_amd import tune, workloads as W
+
+    rng = np.random.default_rng(80 + loop)
+    cases = [([2, 1, 0, 1], True, False), ([3, 2, 2, 1], False, True), ([2, 2, 2, 2, 2, 2], True, False),
+             ([5, 1, 0, 2, 3, 1], True, True), ([4, 3, 5, 2, 6, 4], True, False), ([1, 3, 1, 2, 2, 1], False, False),
+             ([3, 2, 4, 1, 2, 3, 1, 2], True, True), ([2, 2, 2, 2, 2, 2, 2, 2], True, False)]
+    reqs = []
+    for rpt, glynn, real in cases:
+        n = len(rpt)
+        A = random_symmetric(rng, n, complex_=not real, scale=1 / np.sqrt(n))
+        mu = rng.normal(size=n) + (0 if real else 1j * rng.normal(size=n))
+        reqs.append((A, rpt, mu, glynn))
+    for A, rpt, mu in W.config4(count=4, seed=140 + loop):
+        reqs.append((A, rpt, mu, True))
+    preps = [B.prepare_repeated(A, rpt, mu=mu, loop=loop, glynn=g) for A, rpt, mu, g in reqs]
+    twoE = [2 * len(p.job.edge_reps) for p in preps]
+    truths = [O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, glynn=g, precision="dd") for A, rpt, mu, g in reqs]
+    bars = [max(TOL, 2 * relerr(O.hafnian_sieve(A, loop=loop, rpt=rpt, mu=mu, glynn=g, precision="double"), t))
+            for (A, rpt, mu, g), t in zip(reqs, truths)]
+    runs = [dict()] + [dict(tile=s) for s in (1, 3, 2)] + [dict(tile=3, waves=w) for w in (1, 2, 3, 4)]
+    seen = set()
+    for kw in runs:
+        idx = [i for i, e in enumerate(twoE) if e <= REP_SIDE_CAP[kw.get("tile", 2)]]
+        db = B.DeviceBatch([preps[i] for i in idx], **kw)
+        db.run()
+        res = db.results()
+        db.close()
+        for i, r in zip(idx, res):
+            side = kw.get("tile", tune.rep_side(twoE[i]))
+            seen.add(side)
+            print("rep side %s 2E=%d rpt=%s loop=%s: %.2e (bar %.2e)" % (kw, twoE[i], reqs[i][1], loop,
+                                                                        relerr(r, truths[i]), bars[i]), flush=True)
+            assert relerr(r, truths[i]) < bars[i], (kw, reqs[i][1], relerr(r, truths[i]), bars[i])
+    assert seen == {1, 2, 3}
+
+
 @pytest.mark.parametrize("loop", [False, True])
 def test_forced_tile_sides_and_waves(loop):
     """The symmetric matrix-core kernel at every tile side 1..8 (the sides 7 and 8 serve
@@ -424,6 +471,21 @@ def test_tuned_tile_side_equals_the_tile_option():
     finally:
         for lp, t in before.items():
             L.wh_tune_set_tile(0, 3, lp, t)
+    # repeated rows: side 3 (2E <= 12, four terms per wave) tuned to side 2 (2E <= 16), the larger
+    # capacity under the smaller number
+    from thewalrus_amd import workloads as W
+
+    preps = [B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in W.config4(count=16, seed=141)]
+    preps = [p for p in preps if 8 < 2 * len(p.job.edge_reps) <= 12]
+    assert preps
+    at2, at3 = run(tile=2), run(tile=3)
+    assert np.array_equal(run(), at3)
+    before = L.wh_tune_get_tile(2, 3, 1)
+    try:
+        assert L.wh_tune_set_tile(2, 3, 1, 2) == 0
+        assert np.array_equal(run(), at2)
+    finally:
+        L.wh_tune_set_tile(2, 3, 1, before)
 
 
 def test_tile_and_waves_that_do_not_fit_are_refused():

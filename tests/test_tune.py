@@ -80,7 +80,7 @@ def test_waves_round_trip_and_range(restore_table):
     try:
         assert L.wh_tune_set_waves(0, 5, 1, 2) == 0 and L.wh_tune_get_waves(0, 5, 1) == 2
         assert L.wh_tune_set_waves(2, 2, 0, 4) == 0 and L.wh_tune_get_waves(2, 2, 0) == 4
-        for args in ((1, 0, 0, 1), (0, 7, 0, 1), (0, 3, 0, 5), (2, 3, 0, 1), (0, 3, 0, -1)):
+        for args in ((1, 0, 0, 1), (0, 7, 0, 1), (0, 3, 0, 5), (2, 4, 0, 1), (0, 3, 0, -1)):
             assert L.wh_tune_set_waves(*args) != 0
         assert L.wh_tune_get_waves(1, 0, 0) == -1
     finally:
@@ -100,9 +100,13 @@ def test_tile_round_trip_and_range(restore_table):
         assert L.wh_tune_set_tile(0, 3, 1, 4) == 0 and L.wh_tune_get_tile(0, 3, 1) == 4
         assert L.wh_tune_set_tile(0, 5, 0, 8) == 0 and L.wh_tune_get_tile(0, 5, 0) == 8
         assert L.wh_tune_set_tile(2, 1, 0, 2) == 0 and L.wh_tune_get_tile(2, 1, 0) == 2
+        assert L.wh_tune_set_tile(2, 1, 0, 3) == 0 and L.wh_tune_get_tile(2, 1, 0) == 3
+        assert L.wh_tune_set_tile(2, 3, 1, 2) == 0 and L.wh_tune_get_tile(2, 3, 1) == 2
         assert L.wh_tune_set_tile(0, 3, 1, 0) == 0 and L.wh_tune_get_tile(0, 3, 1) == 0
-        # a side not above the class's own, past the kernels' largest, a general-kernel class
-        for args in ((0, 3, 0, 3), (0, 3, 0, 2), (0, 5, 0, 9), (2, 1, 0, 3), (2, 2, 0, 3), (1, 0, 0, 2)):
+        # a side not above the class's own (repeated rows: by the 2E it holds), past the kernels'
+        # largest, a general-kernel class
+        for args in ((0, 3, 0, 3), (0, 3, 0, 2), (0, 5, 0, 9), (2, 3, 0, 1), (2, 3, 0, 3), (2, 2, 0, 3),
+                     (2, 1, 0, 4), (1, 0, 0, 2)):
             assert L.wh_tune_set_tile(*args) != 0
         assert L.wh_tune_get_tile(1, 0, 0) == -1
     finally:

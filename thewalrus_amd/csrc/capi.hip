@@ -209,7 +209,7 @@ enum Kind { KIND_PM1 = 0, KIND_GENERIC = 1, KIND_PM1M = 2, KIND_PM1S = 3, KIND_R
 static bool all_ones_kind(int k) { return k == KIND_PM1 || k == KIND_PM1M || k == KIND_PM1S; }
 constexpr int kPm1MaxT = 6;   // tile sides of the vector kernel and of the tuning table
 constexpr int kPm1sMaxT = 8;  // the symmetric matrix-core kernel also takes n in [50, 64]
-constexpr int kRepMaxT = 2;   // REP: 2E <= 16
+constexpr int kRepMaxT = 3;   // REP tile sides: 1 (2E <= 8), 3 (2E <= 12, four terms per wave), 2 (2E <= 16)
 constexpr int kRepMaxH = 16;  // REP: h = N / 2 <= 16
 
 struct GroupKey {
@@ -271,10 +271,13 @@ size_t group_lds_bytes(const GroupKey& g, int nw) {
 // (kind < 0) when an override does not fit the class.
 int tuned_tile(const GroupKey& g);
 
+// the largest 2E a repeated-row tile side takes
+int rep_cap(int T) { return T == 1 ? 8 : T == 3 ? 12 : T == 2 ? 16 : 0; }
+
 bool tile_fits(const GroupKey& g, int tile) {
     return all_ones_kind(g.kind) ? tile >= g.T && tile <= (g.kind == KIND_PM1S ? kPm1sMaxT : kPm1MaxT) &&
                                        !(g.kind == KIND_PM1M && tile > 5)
-           : g.kind == KIND_REP ? tile >= g.T && tile <= kRepMaxT
+           : g.kind == KIND_REP ? tile >= 1 && tile <= kRepMaxT && rep_cap(tile) >= rep_cap(g.T)
                                 : false;
 }
 
@@ -289,8 +292,8 @@ GroupKey classify(const wh_job_dev& j, int options) {
         }
         g.T = tile;
     } else {  // the tuned tile side of the class, where one is set and the class's kernel takes it
-        const int tt = tuned_tile(g);
-        if (tt > g.T && tile_fits(g, tt)) g.T = tt;
+        const int tt = tuned_tile(g);  // (repeated-row sides order by the 2E they hold: 1, 3, 2)
+        if (tt != 0 && tt != g.T && tile_fits(g, tt)) g.T = tt;
     }
     if (waves) {
         if (!(all_ones_kind(g.kind) || g.kind == KIND_REP) || waves > 4 || group_lds_bytes(g, waves) > 160 * 1024) {
@@ -338,10 +341,10 @@ GroupKey classify_auto(const wh_job_dev& j, int options) {
         return g;
     }
     const int h = j.N / 2;
-    if (!odd && T <= kRepMaxT && h <= kRepMaxH && j.steps < ((int64_t)1 << 31) &&
+    if (!odd && 2 * j.E <= 16 && h <= kRepMaxH && j.steps < ((int64_t)1 << 31) &&
         !(j.flags & WH_F_CHARPOLY) && !(options & WH_OPT_FULL)) {
         g.kind = KIND_REP;
-        g.T = T;
+        g.T = 2 * j.E <= 8 ? 1 : 2 * j.E <= 12 ? 3 : 2;
         g.H = h <= 8 ? 8 : 16;
         g.real = (j.flags & WH_F_REAL) ? 1 : 0;
         g.loop = (j.flags & WH_F_LOOP) ? 1 : 0;
@@ -414,6 +417,13 @@ int64_t chunk_for(const wh_job_dev& j, const GroupKey& g, int nw) {
         if (c < 4 * nw) c = 4 * nw;
         if (c > 4096) c = 4096;
         c = (c + nw - 1) / nw * nw;
+        return c;
+    }
+    if (g.kind == KIND_REP && g.T == 3) {  // each wave takes four terms at a time
+        int64_t c = j.steps / (tb > 0 ? tb : 64);
+        if (c < 16 * nw) c = 16 * nw;
+        if (c > 4096) c = 4096;
+        c = (c + 4 * nw - 1) / (4 * nw) * (4 * nw);
         return c;
     }
     if (g.kind == KIND_REP) {  // each wave takes every nw-th term of its workgroup's chunk
@@ -904,7 +914,9 @@ int wh_tune_get_waves(int kind, int T, int loop) {
 
 int wh_tune_set_tile(int kind, int T, int loop, int tile) {
     if (kind < 0 || kind > 2 || kind == 1 || loop < 0 || loop > 1 || (kind == 0 && (T < 1 || T > kPm1MaxT)) ||
-        (kind == 2 && (T < 1 || T > kRepMaxT)) || (tile != 0 && (tile <= T || tile > (kind == 0 ? kPm1sMaxT : kRepMaxT))))
+        (kind == 2 && (T < 1 || T > kRepMaxT)) ||
+        (tile != 0 && (kind == 0 ? tile <= T || tile > kPm1sMaxT
+                                 : tile < 1 || tile > kRepMaxT || rep_cap(tile) <= rep_cap(T))))
         return fail(WH_EINVAL, "tile entry out of range (kind %d, T %d, loop %d, tile %d)", kind, T, loop, tile);
     g_tune_tile[kind][T][loop].store(tile, std::memory_order_relaxed);
     return WH_OK;

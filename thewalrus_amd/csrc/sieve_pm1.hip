@@ -776,6 +776,351 @@ __device__ __forceinline__ void rep_body(const wh_job_dev& J, const cplx* __rest
     }
 }
 
+// ---- Repeated rows with 2E <= 12, four terms per wave.  One term on a whole wave (rep_body)
+// pads 2E = 10 or 12 to 16 and spends most of its instructions on per-power reductions over 64
+// lanes and on coefficient batches that leave 56 lanes idle.  Here each 16-lane group of the wave
+// takes its own term: the group is a 4 x 4 grid of lanes, each owning a 3 x 3 tile of the 12 x 12
+// padded power (its own matrix in LDS), so the products pad to 12, and the per-power sums run
+// within the 16 lanes for the four terms at once.  Everything else (digits, delta, prefactor,
+// the half-products powers, loop vectors, coefficient) is rep_body's, per group.
+#define WH_DPP_ROR4 0x124  // lane i <- i + 4 within 16
+constexpr int kR4N = 12;      // padded side
+constexpr int kR4S = 13;      // LDS row stride (elements)
+constexpr int kR4Iters = 2;   // wave iterations (of four terms) per coefficient batch
+constexpr int kR4Slots = 4 * kR4Iters;
+
+__device__ __forceinline__ double group16_sum(double v) {
+    v += dpp_d<WH_DPP_XOR1>(v);
+    v += dpp_d<WH_DPP_XOR2>(v);
+    v += dpp_d<WH_DPP_HALF_MIRROR>(v);
+    return v + dpp_d<WH_DPP_ROR8>(v);
+}
+__device__ __forceinline__ double col4_sum(double v) {  // lanes i, i + 4, i + 8, i + 12 of the 16
+    v += dpp_d<WH_DPP_ROR4>(v);
+    return v + dpp_d<WH_DPP_ROR8>(v);
+}
+__device__ __forceinline__ double row4_sum(double v) {  // lanes 4q .. 4q + 3 of the 16
+    v += dpp_d<WH_DPP_XOR1>(v);
+    return v + dpp_d<WH_DPP_XOR2>(v);
+}
+__device__ __forceinline__ cplx group16_e(cplx v) { return cmk(group16_sum(v.re), group16_sum(v.im)); }
+__device__ __forceinline__ double group16_e(double v) { return group16_sum(v); }
+__device__ __forceinline__ cplx col4_e(cplx v) { return cmk(col4_sum(v.re), col4_sum(v.im)); }
+__device__ __forceinline__ double col4_e(double v) { return col4_sum(v); }
+__device__ __forceinline__ cplx row4_e(cplx v) { return cmk(row4_sum(v.re), row4_sum(v.im)); }
+__device__ __forceinline__ double row4_e(double v) { return row4_sum(v); }
+
+// Eight sums over each 16-lane group at once: lane i returns the group's sum of v[(i >> 1) & 7].
+// Each stage halves the values a lane carries: partners i ^ 8 (bit 3 picks the half kept),
+// 7 - i (bit 2), i ^ 2 (bit 1), then i ^ 1; the 16 partner masks are all distinct.
+__device__ __forceinline__ double group16_sum8(const double (&v)[8], int gl) {
+    const bool b3 = (gl >> 3) & 1, b2 = (gl >> 2) & 1, b1 = (gl >> 1) & 1;
+    double s1[4], s2[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double keep = b3 ? v[k + 4] : v[k], send = b3 ? v[k] : v[k + 4];
+        s1[k] = keep + dpp_d<WH_DPP_ROR8>(send);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double keep = b2 ? s1[k + 2] : s1[k], send = b2 ? s1[k] : s1[k + 2];
+        s2[k] = keep + dpp_d<WH_DPP_HALF_MIRROR>(send);
+    }
+    const double keep = b1 ? s2[1] : s2[0], send = b1 ? s2[0] : s2[1];
+    const double s3 = keep + dpp_d<WH_DPP_XOR2>(send);
+    return s3 + dpp_d<WH_DPP_XOR1>(s3);
+}
+
+// LDS layout (element type E): M, XD0, D, then per wave four powers, then per wave the term slots
+// (complex), the per-wave results and the binomial table.
+__host__ __device__ constexpr size_t rep4_align32(size_t x) { return (x + 31) / 32 * 32; }
+__host__ __device__ constexpr size_t rep4_slots_off(size_t esz, int nw) {
+    return rep4_align32(((size_t)kR4N * kR4S + 2 * kR4N + (size_t)nw * 4 * kR4N * kR4S) * esz);
+}
+__host__ __device__ constexpr size_t rep4_wres_off(size_t esz, int nw, int H) {
+    return rep4_align32(rep4_slots_off(esz, nw) + (size_t)nw * kR4Slots * 2 * rep_ptn(H) * sizeof(cplx));
+}
+__host__ __device__ constexpr size_t rep4_lds_bytes(size_t esz, int nw, int H) {
+    return rep4_wres_off(esz, nw, H) + (size_t)nw * sizeof(double4) + (size_t)(kR4N / 2) * (H + 1) * sizeof(double);
+}
+
+template <bool REAL, bool LOOP, int H>
+__device__ __forceinline__ void rep4_body(const wh_job_dev& J, const cplx* __restrict__ pool,
+                                          double4* __restrict__ partials, int64_t b, unsigned char* smem) {
+    using E = typename Elem<REAL>::T;
+    constexpr int N = kR4N, S = kR4S, PTN = rep_ptn(H), NI = kR4Iters;
+    const int nw = blockDim.x / WH_WAVE;
+    const int wave = threadIdx.x / WH_WAVE;
+    const int lane = threadIdx.x % WH_WAVE;
+    const int grp = lane >> 4, gl = lane & 15, gbase = lane & ~15;
+    const int ti = gl >> 2, tj = gl & 3;
+    const int I0 = 3 * ti, J0 = 3 * tj;
+    const int ne = J.E;
+    const int n = 2 * ne;
+    const int h = J.N / 2;
+    const bool glynn = (J.flags & WH_F_GLYNN) != 0;
+
+    E* Ms = reinterpret_cast<E*>(smem);
+    E* XD0 = Ms + N * S;
+    E* Dv = XD0 + N;
+    E* Pw = Dv + N + (wave * 4 + grp) * N * S;  // this group's power
+    cplx* slots = reinterpret_cast<cplx*>(smem + rep4_slots_off(sizeof(E), nw)) + wave * kR4Slots * 2 * PTN;
+    double4* wres = reinterpret_cast<double4*>(smem + rep4_wres_off(sizeof(E), nw, H));
+    double* btab = reinterpret_cast<double*>(wres + nw);
+
+    const cplx* A = pool + J.a_off;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        int r = e / N, c = e - r * N;
+        E v;
+        ezero(v);
+        if (r < n && c < n) {
+            int src = c < ne ? c + ne : c - ne;
+            v = eload(A, (int64_t)r * n + src, (E*)nullptr);
+        }
+        Ms[r * S + c] = v;
+    }
+    if (LOOP) {
+        const cplx* Dg = pool + J.d_off;
+        for (int e = threadIdx.x; e < N; e += blockDim.x) {
+            E xd, dd;
+            ezero(xd);
+            ezero(dd);
+            if (e < n) {
+                int src = e < ne ? e + ne : e - ne;  // (X D)[e]
+                xd = eload(Dg, src, (E*)nullptr);
+                dd = eload(Dg, e, (E*)nullptr);
+            }
+            XD0[e] = xd;
+            Dv[e] = dd;
+        }
+    }
+    for (int e = threadIdx.x; e < ne * (H + 1); e += blockDim.x) {
+        int i = e / (H + 1), k = e - i * (H + 1);
+        btab[e] = binom_d(J.e[i], k);
+    }
+    __syncthreads();
+
+    const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
+    int64_t t1 = t0 + J.chunk_terms;
+    if (t1 > J.steps) t1 = J.steps;
+    // lane gl < ne of each group owns digit gl: its base and the weight of the digits after it
+    const int my_e = gl < ne ? J.e[gl] : 0;
+    uint32_t my_w = 1;
+    for (int l = gl + 1; l < ne; ++l) my_w *= (uint32_t)(J.e[l] + 1);
+    int srcJ[3], srcI[3];  // the edge index of the lane's columns and rows (c mod ne; 0 on padding)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int col = J0 + c, row = I0 + c;
+        srcJ[c] = col < ne ? col : col < n ? col - ne : 0;
+        srcI[c] = row < ne ? row : row < n ? row - ne : 0;
+    }
+    const int src_l = gl < ne ? gl : gl < n ? gl - ne : 0;
+
+    csum acc_sum;
+    acc_sum.zero();
+
+    int it = 0;  // iteration within the coefficient batch
+    for (int64_t t = t0 + 4 * wave; t < t1; t += 4 * nw) {
+        // group grp takes term t + grp; past the chunk's end it repeats term t with prefactor 0
+        const bool valid = t + grp < t1;
+        const uint32_t tu = (uint32_t)(valid ? t + grp : t);
+        const int k = gl < ne ? (int)((tu / my_w) % (uint32_t)(my_e + 1)) : 0;
+        const int delta = gl < ne ? (glynn ? 2 * k - my_e : k) : 0;
+        int ksum = k;
+        double bin = gl < ne ? btab[gl * (H + 1) + k] : 1.0;
+#pragma unroll
+        for (int msk = 1; msk < 8; msk <<= 1) {  // ne <= 6: lanes 0..7 of the group hold every digit
+            ksum += __shfl_xor(ksum, msk, 8);
+            bin *= __shfl_xor(bin, msk, 8);  // exact: a product of integers below 2^53
+        }
+        ksum = __shfl(ksum, gbase, WH_WAVE);
+        bin = __shfl(bin, gbase, WH_WAVE);
+        const int delta0 = __shfl(delta, gbase, WH_WAVE);
+        double prefac = (((h - ksum) & 1) ? -1.0 : 1.0) * bin;
+        if (glynn && delta0 == 0) prefac *= 0.5;
+        if (!valid) prefac = 0.0;
+        double sgnJ[3], sgnI[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double dj = (double)__shfl(delta, gbase + srcJ[c], WH_WAVE);
+            const double di = (double)__shfl(delta, gbase + srcI[c], WH_WAVE);
+            sgnJ[c] = J0 + c < n ? dj : 0.0;
+            sgnI[c] = I0 + c < n ? di : 0.0;
+        }
+        const double dl = (double)__shfl(delta, gbase + src_l, WH_WAVE);
+
+        cplx* pt = slots + (it * 4 + grp) * 2 * PTN;  // pt[0] = (prefactor, 0); p[1..h]
+        cplx* lv = pt + PTN;                          // loop scalars v[1..h]
+        E acc[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[r][c] = escale(Ms[(I0 + r) * S + J0 + c], sgnJ[c]);
+        E zprev[3];
+        if (LOOP) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) zprev[r] = Dv[I0 + r];
+        }
+        auto store = [&]() {
+            wave_lds_sync();  // the group's reads of the previous power are done
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) Pw[(I0 + r) * S + J0 + c] = acc[r][c];
+            wave_lds_sync();  // the whole power is stored before any lane reads it
+        };
+        auto tdot = [&]() {  // lane part of <acc, Pw^T>
+            E sd;
+            ezero(sd);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) emac(sd, acc[r][c], Pw[(J0 + c) * S + I0 + r]);
+            return sd;
+        };
+        store();
+        const int Ah = (h + 1) / 2;
+        for (int a = 1; a <= Ah; ++a) {
+            E qo, qe;
+            if (a == 1) {
+                ezero(qo);
+                if (ti == tj) {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) qo = eadd(qo, acc[r][r]);
+                }
+                qe = tdot();
+            } else {
+                // acc = (Pw M) diag(delta), k summed in order; next k's operands load during k's FMAs
+                E nacc[3][3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ezero(nacc[r][c]);
+                E av[3], bv[3], an[3], bn[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) av[r] = Pw[(I0 + r) * S];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) bv[c] = Ms[J0 + c];
+                for (int kk = 0; kk < n; ++kk) {
+                    const int kn = kk + 1 < n ? kk + 1 : kk;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) an[r] = Pw[(I0 + r) * S + kn];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bn[c] = Ms[kn * S + J0 + c];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) emac(nacc[r][c], av[r], bv[c]);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) av[r] = an[r];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bv[c] = bn[c];
+                }
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) acc[r][c] = escale(nacc[r][c], sgnJ[c]);
+                qo = tdot();
+                store();
+                qe = tdot();
+            }
+            E de, dodd;
+            ezero(de);
+            ezero(dodd);
+            E znew[3];
+            if (LOOP) {
+                // y = XD_d B^a (columns, over the group's 4 row tiles), z = B^a D (rows, over its
+                // 4 column tiles); the dots pair y and z on the diagonal tiles
+                E y[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ezero(y[c]);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) emac(y[c], escale(XD0[I0 + r], sgnI[r]), acc[r][c]);
+                    y[c] = col4_e(y[c]);
+                }
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    ezero(znew[r]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) emac(znew[r], acc[r][c], Dv[J0 + c]);
+                    znew[r] = row4_e(znew[r]);
+                }
+                if (ti == tj) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        emac(de, y[c], zprev[c]);
+                        emac(dodd, y[c], znew[c]);
+                    }
+                }
+            }
+            const cplx qoc = etoc(qo), qec = etoc(qe), dec = etoc(de), doc = etoc(dodd);
+            const double v8[8] = {qoc.re, qoc.im, qec.re, qec.im, dec.re, dec.im, doc.re, doc.im};
+            const double sum8 = group16_sum8(v8, gl);
+            const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+            if ((gl & 1) == 0) {  // lane 2k of the group holds the sum of v8[k]
+                const int kv = gl >> 1;
+                double* dst = nullptr;
+                if (kv < 2) dst = &pt[ko].re;
+                else if (kv < 4) dst = &pt[ke].re;  // ke <= h + 1 < PTN
+                else if (LOOP && kv < 6 && 2 * a <= h) dst = &lv[2 * a].re;
+                else if (LOOP && kv >= 6 && 2 * a + 1 <= h) dst = &lv[2 * a + 1].re;
+                if (dst) dst[kv & 1] = sum8;
+            }
+            if (LOOP) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) zprev[r] = znew[r];
+            }
+        }
+        if (LOOP) {
+            // v_1 = XD_d . D (lane gl < n of the group holds index gl)
+            E part;
+            ezero(part);
+            if (gl < n) emac(part, escale(XD0[gl], dl), Dv[gl]);
+            const cplx v1 = etoc(group16_e(part));
+            if (gl == 0) lv[1] = v1;
+        }
+        if (gl == 0) pt[0] = cmk(prefac, 0.0);
+        ++it;
+        if (it == NI || t + 4 * nw >= t1) {
+            wave_lds_sync();  // every group's slots as their lanes stored them
+            if (lane < 4 * it) {  // lane j: slot j (iteration j / 4, group j % 4)
+                cplx* ptj = slots + lane * 2 * PTN;
+                const double pf = ptj[0].re;
+                if (pf != 0.0) {
+                    const E top = coeff_top_lds<E, H, LOOP>(ptj, ptj + PTN, h);
+                    acc_sum.add(cmulr(etoc(top), pf));
+                }
+            }
+            wave_lds_sync();  // the slots are free again
+            it = 0;
+        }
+    }
+
+    double4* lsum = reinterpret_cast<double4*>(slots);
+    if (lane < kR4Slots) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    wave_lds_sync();
+    if (lane == 0) {
+        csum sm;
+        sm.zero();
+        for (int j = 0; j < kR4Slots; ++j) {
+            double4 pq = lsum[j];
+            sm.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        wres[wave] = make_double4(sm.sr, sm.cr, sm.si, sm.ci);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        csum sm;
+        sm.zero();
+        for (int w = 0; w < nw; ++w) {
+            double4 pq = wres[w];
+            sm.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        partials[b] = make_double4(sm.sr, sm.cr, sm.si, sm.ci);
+    }
+}
+
 }  // namespace
 
 template <int T, bool REAL, bool LOOP, bool HALF>
@@ -865,7 +1210,46 @@ wh_sieve_rep(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restr
     rep_body<T, REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
 }
 
+template <bool REAL, bool LOOP, int H>
+__global__ void __launch_bounds__(256)
+wh_sieve_rep4(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
+              double4* __restrict__ partials, int64_t block_offset) {
+    extern __shared__ __align__(32) unsigned char wh_smem[];
+    const int64_t b = blockIdx.x + block_offset;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
+    }
+    rep4_body<REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
+}
+
+template <bool REAL, bool LOOP, int H>
+static hipError_t launch_rep4_t(const wh_job_dev* jobs, int njobs, const void* pool, void* partials,
+                                int64_t nblocks, int64_t block_offset, int nw, hipStream_t stream) {
+    size_t lds = rep4_lds_bytes(REAL ? sizeof(double) : sizeof(cplx), nw, H);
+    static wh_lds_limit lim;
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_rep4<REAL, LOOP, H>, lds, lim);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((wh_sieve_rep4<REAL, LOOP, H>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream,
+                       jobs, njobs, (const cplx*)pool, (double4*)partials, block_offset);
+    return hipGetLastError();
+}
+
+template <int H>
+static hipError_t launch_rep4_rl(bool real, bool loop, const wh_job_dev* jobs, int njobs, const void* pool,
+                                 void* partials, int64_t nblocks, int64_t off, int nw, hipStream_t s) {
+    if (real) {
+        if (loop) return launch_rep4_t<true, true, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+        return launch_rep4_t<true, false, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    }
+    if (loop) return launch_rep4_t<false, true, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+    return launch_rep4_t<false, false, H>(jobs, njobs, pool, partials, nblocks, off, nw, s);
+}
+
+// Tile side 3 of the repeated-row kernel is the four-terms-per-wave layout (2E <= 12).
 size_t wh_rep_lds_bytes(int T, bool real, int nw, int H) {
+    if (T == 3) return rep4_lds_bytes(real ? sizeof(double) : sizeof(cplx), nw, H);
     size_t esz = real ? sizeof(double) : sizeof(cplx);
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1_PAD;
     return (1 + nw) * NP * S * esz + 2 * NP * esz + nw * (size_t)(rep_nb() * 2 * rep_ptn(H)) * sizeof(cplx) +
@@ -901,5 +1285,7 @@ hipError_t wh_launch_rep(int T, int H, bool real, bool loop, const wh_job_dev* j
     if (T == 1 && H == 16) return launch_rep_rl<1, 16>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
     if (T == 2 && H == 8) return launch_rep_rl<2, 8>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
     if (T == 2 && H == 16) return launch_rep_rl<2, 16>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    if (T == 3 && H == 8) return launch_rep4_rl<8>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
+    if (T == 3 && H == 16) return launch_rep4_rl<16>(real, loop, jobs, njobs, pool, partials, nblocks, block_offset, nw, stream);
     return hipErrorInvalidValue;
 }

@@ -11,15 +11,15 @@ the built-in default by more than 1% (`choose`):
   (n_pad = 8T), plain and loop;
 * kind 1: the general kernel (odd N, and repeated rows beyond the repeated-row kernel's
   limits), plain and loop;
-* kind 2: the repeated-row kernel (even N, 2E <= 16, h <= 16), by tile side class T = 1, 2,
-  plain and loop.
+* kind 2: the repeated-row kernel (even N, 2E <= 16, h <= 16), by tile side class T = 1
+  (2E <= 8), 3 (2E <= 12, four terms per wave) and 2 (2E <= 16), plain and loop.
 
 For the all-ones and repeated-row classes the tuner then times the waves per workgroup, 1 to 4
 against the built-in choice (the count that fits the most waves per CU in the LDS), and keeps a
 count only when it beats that choice by more than 1% (`wh_tune_set_waves`; a count whose
 workgroup does not fit the LDS is ignored by the library).  Last it times the next larger tile
-side for those classes (all-ones T = 1..5, repeated-row T = 1: the same requests with more zero
-padding) and keeps it only when it wins by more than 1% (`wh_tune_set_tile`).
+side for those classes (all-ones T = 1..5, repeated-row sides 1 and 3: the same requests with
+more zero padding) and keeps it only when it wins by more than 1% (`wh_tune_set_tile`).
 
 The table lives in a JSON file (`tuning.json` next to this module by default, or the path in
 the environment variable ``WH_TUNING``; ``WH_TUNING=none`` keeps the built-in defaults) that the
@@ -70,7 +70,7 @@ def current(lib=None):
     """The library's table: {(kind, T, loop): blocks_per_request}, 0 = built-in default."""
     L = lib or _native.load()
     out = {}
-    for kind, Ts in ((0, range(1, 7)), (1, (0,)), (2, (1, 2))):
+    for kind, Ts in ((0, range(1, 7)), (1, (0,)), (2, (1, 2, 3))):
         for T in Ts:
             for loop in (0, 1):
                 out[(kind, T, loop)] = int(L.wh_tune_get(kind, T, loop))
@@ -101,6 +101,16 @@ def load(path=None, lib=None):
     return entries
 
 
+def rep_side(two_e):
+    """The repeated-row kernel's tile side for a term matrix of side 2E: 1 (2E <= 8), 3 (<= 12,
+    four terms per wave) or 2 (<= 16)."""
+    return 1 if two_e <= 8 else 3 if two_e <= 12 else 2
+
+
+# the next larger tile side the tuner tries per class (repeated-row sides by the 2E they hold)
+NEXT_TILE = dict([((0, T), T + 1) for T in range(1, 6)] + [((2, 1), 3), ((2, 3), 2)])
+
+
 def _class_workload(kind, T, loop, rng):
     """Representative requests of one kernel class: about 2^18 terms in total (at most 256
     requests), complex CN(0, 1/n) symmetric matrices."""
@@ -116,14 +126,16 @@ def _class_workload(kind, T, loop, rng):
         count = int(min(256, max(1, 2 ** 18 // 2 ** (h - 1))))
         return [B.prepare(sym(n), loop=bool(loop)) for _ in range(count)]
     if kind == 2:
-        # repeated-row kernel: 32 photons on 4 modes (2E <= 8, T = 1) or 8 modes (2E in 10..16, T = 2)
-        modes = 4 if T == 1 else 8
+        # repeated-row kernel, 32 photons: 4 modes (2E <= 8, side 1), 8 modes (2E in 10..12,
+        # side 3: four terms per wave) or 16 modes (2E in 14..16, side 2; fewer requests, as each
+        # has tens of thousands of terms)
+        modes, count = {1: (4, 256), 3: (8, 256), 2: (16, 32)}[T]
         preps = []
-        while len(preps) < 256:
+        while len(preps) < count:
             rpt = rng.multinomial(32, [1 / modes] * modes)
             mu = (rng.normal(size=modes) + 1j * rng.normal(size=modes)) if loop else None
             p = B.prepare_repeated(sym(modes), rpt, mu=mu, loop=bool(loop))
-            if p.on_device and (2 * len(p.job.edge_reps) + 7) // 8 == T:
+            if p.on_device and rep_side(2 * len(p.job.edge_reps)) == T:
                 preps.append(p)
         return preps
     # general kernel: repeated rows past the repeated-row kernel's h <= 16, 8 modes, 40 photons
@@ -184,7 +196,7 @@ def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234
     rng = np.random.default_rng(seed)
     if classes is None:
         classes = ([(0, T, loop) for T in range(1, 7) for loop in (0, 1)] + [(1, 0, 0), (1, 0, 1)]
-                   + [(2, T, loop) for T in (1, 2) for loop in (0, 1)])
+                   + [(2, T, loop) for T in (1, 3, 2) for loop in (0, 1)])
     stream = torch.cuda.Stream()
     entries = []
     for kind, T, loop in classes:
@@ -206,9 +218,9 @@ def autotune(path=DEFAULT_PATH, candidates=None, reps=3, classes=None, seed=1234
             _native.check(L.wh_tune_set_waves(kind, T, loop, int(wbest)))
             e["waves"] = int(wbest)
             e["waves_ms"] = {str(w): round(1e3 * v, 4) for w, v in wtimes.items()}
-            if (kind == 0 and T <= 5) or (kind == 2 and T == 1):
+            if (kind, T) in NEXT_TILE:
                 ttimes = {}
-                for tl in (0, T + 1):
+                for tl in (0, NEXT_TILE[(kind, T)]):
                     _native.check(L.wh_tune_set_tile(kind, T, loop, tl))
                     ttimes[tl] = _time(preps, reps, torch, stream)
                 tbest = choose_waves(ttimes)
