@@ -28,7 +28,10 @@ The same JSON line also carries:
     one launch; 3: n = 40 loop single solve; 4: 1024 hafnian_repeated requests);
   * next_rows: the GBS samplers' batched loop hafnians (loop_hafnian_batch, _gamma) and the
     montrealer at 16 and 20 modes, per call, with the CPU oracle beside them;
-  * roofline: FP64 product flops of the dominant kernel / its HIP-event time (same stream);
+  * roofline: FP64 product flops of the dominant kernel / its HIP-event time (same stream); its
+    HBM traffic per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of
+    ``--roofline-only`` run as child processes, copy-calibrated (the committed counter summary
+    when rocprofv3 is absent or with --no-live-traffic);
   * cpu_baseline: the CPU restatement of the reference algorithm on host cores, timed per n on
     a bounded sample of terms, summed over the stream's requests.
 """
@@ -341,6 +344,56 @@ def counter_traffic(name):
     return None
 
 
+def live_traffic(loop, timeout_s=120):
+    """HBM bytes per launch of the dominant kernel measured in this run: two rocprofv3 --pmc
+    child runs of ``bench.py --roofline-only --calibrate-copy-mb 256`` (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch, averaged over the kernel's launches), each scaled by the
+    reading of the same counter for the child's 256 MiB device-to-device copy (the bytes the copy
+    reads, resp. writes, over its counter).  None when rocprofv3 is absent or a pass fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    want = "wh_sieve_pm1s<5, %s" % ("true" if loop else "false")
+    copy_bytes = 256 << 20
+    got = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            od = os.path.join(d, ctr.lower())
+            cmd = [exe, "--pmc", ctr, "--kernel-trace", "--output-format", "csv", "-d", od, "-o", "p", "--",
+                   sys.executable, os.path.abspath(__file__), "--roofline-only", "--calibrate-copy-mb", "256"]
+            try:
+                r = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                   timeout=timeout_s)
+            except (subprocess.TimeoutExpired, OSError):
+                return None
+            path = os.path.join(od, "p_counter_collection.csv")
+            if not os.path.exists(path):
+                return None
+            per = {}
+            with open(path, newline="") as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != ctr:
+                        continue
+                    key = (row["Dispatch_Id"], row["Kernel_Name"])
+                    per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+            copies = [v for (_, name), v in per.items() if "copyBuffer" in name]
+            kern = [v for (_, name), v in per.items() if name.startswith("void " + want)]
+            if r.returncode not in (0, -11, 139) or not copies or not kern or max(copies) <= 0:
+                return None
+            got[ctr] = (sum(kern) / len(kern), max(copies))
+    fetch, fcopy = got["FETCH_SIZE"]
+    write, wcopy = got["WRITE_SIZE"]
+    ff, wf = copy_bytes / (1024.0 * fcopy), copy_bytes / (1024.0 * wcopy)
+    return {"bytes_per_launch": fetch * 1024 * ff + write * 1024 * wf, "fetch_kib": fetch, "write_kib": write,
+            "fetch_scale": ff, "write_scale": wf}
+
+
 def w_rep(job):
     """The reference's product flops for one prepared repeated-row request: over its terms
     (mixed-radix digits, _hafnian.py:162-180), 8 m^3 max(min(h+1, m) - 2, 0) with m = 2 nnz(delta)
@@ -509,6 +562,9 @@ def main():
                     "batches of 500 with the persistent launch off, on (default rule) and forced")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-live-traffic", action="store_true",
+                    help="take the roofline's traffic from the committed counter summary instead of two "
+                    "rocprofv3 --pmc passes in this run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -568,6 +624,14 @@ def main():
     _, stats_half, _ = serve_stream(half, 0.5 * capacity, Server, seed=143)
 
     roof = dominant_kernel(reqs[:R], B, torch, stream)
+    if roof:
+        roof["traffic_source"] = "profiles/%s_counters.md" % ("pm1s_T5_loop" if "loop" in roof["kernel"] else
+                                                               "pm1s_T5_plain")
+        lt = None if args.no_live_traffic else live_traffic("loop" in roof["kernel"])
+        if lt:
+            roof["traffic"] = lt["bytes_per_launch"]
+            roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this run, copy-calibrated"
+            roof["traffic_counters"] = lt
 
     pc = None
     if not args.no_percall:

@@ -330,7 +330,7 @@ constexpr bool pm1s_3m() {
 // without register spills): the trace loads of V_(a_prev) issue at the start of the last chunk
 // (into the registers the operand prefetch no longer needs), and each group's final value
 // (the three-product combination where it applies), its share of tr(B^(2 a_prev + 1)) and its
-// V stores follow two groups behind the group's last MFMA, in the matrix cores' shadow.  The
+// V stores follow three groups behind the group's last MFMA, in the matrix cores' shadow.  The
 // plain form runs them after the product: 13% of the T = 5 loop launch; fused, n = 32..40 run
 // 3.5-4.6% faster and the real n = 24 batch 3.5%.  The complex T = 6 and real T = 5 instances
 // keep the plain form (with the fused one they spill or lose the full unroll of the product),
@@ -399,7 +399,7 @@ __device__ __forceinline__ cplx sym_product_epi(cplx* Vb, const cplx* __restrict
     constexpr int NG = T * (T + 1) / 2;
     constexpr bool G3 = pm1s_3m<T, LOOP, RE>();
     constexpr bool FB = pm1s_fused<T, LOOP, RE>();
-    constexpr int LAG = 2;  // groups between a group's last MFMA and its boundary step
+    constexpr int LAG = 3;  // groups between a group's last MFMA and its boundary step (2: 0.6% slower at T = 5)
     BndS<T> Bd;
     Bd.sd = Bd.so = Bd.s = cmk(0.0, 0.0);
     double nY[T][T];  // G3: sum of Im a Im b
