@@ -1121,6 +1121,354 @@ __device__ __forceinline__ void rep4_body(const wh_job_dev& J, const cplx* __res
     }
 }
 
+// ---- The same four terms per wave with the products on the FP64 matrix cores (WH_REP4_MFMA,
+// on by default).  v_mfma_f64_4x4x4f64 forms four independent 4 x 4 x 4 block products, block b
+// on the lanes with (lane >> 2) & 3 == b: with x = lane & 3 and y = lane >> 4, a lane supplies
+// A[x][y] and B[y][x] of its block and receives D[y][x].  Term g of the wave takes block g, so
+// one instruction advances the four terms at once.  A term's 12 x 12 power is 3 x 3 blocks of
+// 4 x 4, and lane (x, y) of block g holds P[4 bi + y][4 bj + x] for every (bi, bj); a product is
+// 3 k chunks x 9 blocks, each complex block product three real MFMAs in Gauss's form
+// (Re = X - Y, Im = Z - X - Y with X = Re a Re b, Y = Im a Im b, Z = (Re a + Im a)(Re b + Im b)),
+// one for real inputs.  Sums over a term's 16 lanes run over lane bits 0, 1 (DPP) and 4, 5
+// (permlane swaps).  Digits, delta, prefactor, powers, loop dots and the coefficient are
+// rep4_body's.
+#ifndef WH_REP4_MFMA
+#define WH_REP4_MFMA 1
+#endif
+
+__device__ __forceinline__ double r4m_mfma(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+// sum over the 16 lanes of a term (lane bits 0, 1, 4, 5)
+__device__ __forceinline__ double term16_sum(double v) {
+    v += dpp_d<WH_DPP_XOR1>(v);
+    v += dpp_d<WH_DPP_XOR2>(v);
+    v = swap16_sum(v);
+    return swap32_sum(v);
+}
+__device__ __forceinline__ cplx term16_e(cplx v) { return cmk(term16_sum(v.re), term16_sum(v.im)); }
+__device__ __forceinline__ double term16_e(double v) { return term16_sum(v); }
+// over lane bits 4, 5 (the lanes of a term with the same x: one column)
+__device__ __forceinline__ cplx ycol_e(cplx v) { return cmk(swap32_sum(swap16_sum(v.re)), swap32_sum(swap16_sum(v.im))); }
+__device__ __forceinline__ double ycol_e(double v) { return swap32_sum(swap16_sum(v)); }
+// over lane bits 0, 1 (the lanes of a term with the same y: one row)
+__device__ __forceinline__ cplx xrow_e(cplx v) { return row4_e(v); }
+__device__ __forceinline__ double xrow_e(double v) { return row4_e(v); }
+
+// Eight sums over each term's 16 lanes at once: lane l returns the sum of v[((l >> 1) & 1) |
+// (((l >> 4) & 3) << 1)] (halving stages over lane bits 5, 4, 1, then 0).
+__device__ __forceinline__ double term16_sum8(const double (&v)[8], int lane) {
+    double s1[4], s2[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s1[k] = dswap32(v[k], v[k + 4]);  // value k + 4 bit5
+#pragma unroll
+    for (int k = 0; k < 2; ++k) s2[k] = dswap16(s1[k], s1[k + 2]);  // value k + 2 bit4 + 4 bit5
+    const bool b1 = (lane >> 1) & 1;
+    const double keep = b1 ? s2[1] : s2[0], send = b1 ? s2[0] : s2[1];
+    const double s3 = keep + dpp_d<WH_DPP_XOR2>(send);  // value bit1 + 2 bit4 + 4 bit5
+    return s3 + dpp_d<WH_DPP_XOR1>(s3);
+}
+
+template <bool REAL, bool LOOP, int H>
+__device__ __forceinline__ void rep4m_body(const wh_job_dev& J, const cplx* __restrict__ pool,
+                                           double4* __restrict__ partials, int64_t b, unsigned char* smem) {
+    using E = typename Elem<REAL>::T;
+    constexpr int N = kR4N, S = kR4S, PTN = rep_ptn(H), NI = kR4Iters;
+    const int nw = blockDim.x / WH_WAVE;
+    const int wave = threadIdx.x / WH_WAVE;
+    const int lane = threadIdx.x % WH_WAVE;
+    const int grp = (lane >> 2) & 3;  // the term (MFMA block) of the lane
+    const int x = lane & 3, y = lane >> 4;
+    const int gl = x + 4 * y;         // the lane's index within its term
+    const int gbase = 4 * grp;        // the term's lane with gl = 0
+    const int ne = J.E;
+    const int n = 2 * ne;
+    const int h = J.N / 2;
+    const bool glynn = (J.flags & WH_F_GLYNN) != 0;
+
+    E* Ms = reinterpret_cast<E*>(smem);
+    E* XD0 = Ms + N * S;
+    E* Dv = XD0 + N;
+    E* Pw = Dv + N + (wave * 4 + grp) * N * S;  // this term's power
+    cplx* slots = reinterpret_cast<cplx*>(smem + rep4_slots_off(sizeof(E), nw)) + wave * kR4Slots * 2 * PTN;
+    double4* wres = reinterpret_cast<double4*>(smem + rep4_wres_off(sizeof(E), nw, H));
+    double* btab = reinterpret_cast<double*>(wres + nw);
+
+    const cplx* A = pool + J.a_off;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        int r = e / N, c = e - r * N;
+        E v;
+        ezero(v);
+        if (r < n && c < n) {
+            int src = c < ne ? c + ne : c - ne;
+            v = eload(A, (int64_t)r * n + src, (E*)nullptr);
+        }
+        Ms[r * S + c] = v;
+    }
+    if (LOOP) {
+        const cplx* Dg = pool + J.d_off;
+        for (int e = threadIdx.x; e < N; e += blockDim.x) {
+            E xd, dd;
+            ezero(xd);
+            ezero(dd);
+            if (e < n) {
+                int src = e < ne ? e + ne : e - ne;  // (X D)[e]
+                xd = eload(Dg, src, (E*)nullptr);
+                dd = eload(Dg, e, (E*)nullptr);
+            }
+            XD0[e] = xd;
+            Dv[e] = dd;
+        }
+    }
+    for (int e = threadIdx.x; e < ne * (H + 1); e += blockDim.x) {
+        int i = e / (H + 1), k = e - i * (H + 1);
+        btab[e] = binom_d(J.e[i], k);
+    }
+    __syncthreads();
+
+    const int64_t t0 = (b - J.chunk_base) * J.chunk_terms;
+    int64_t t1 = t0 + J.chunk_terms;
+    if (t1 > J.steps) t1 = J.steps;
+    // lane gl < ne of each term owns digit gl: its base and the weight of the digits after it
+    const int my_e = gl < ne ? J.e[gl] : 0;
+    uint32_t my_w = 1;
+    for (int l = gl + 1; l < ne; ++l) my_w *= (uint32_t)(J.e[l] + 1);
+    // the lane of the term holding digit d (gl = d)
+    auto dlane = [&](int d) { return gbase + (d & 3) + 16 * (d >> 2); };
+    int srcJ[3], srcI[3];  // the edge index of the lane's columns 4 bj + x and rows 4 bi + y
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int col = 4 * c + x, row = 4 * c + y;
+        srcJ[c] = col < ne ? col : col < n ? col - ne : 0;
+        srcI[c] = row < ne ? row : row < n ? row - ne : 0;
+    }
+    const int src_l = gl < ne ? gl : gl < n ? gl - ne : 0;
+
+    csum acc_sum;
+    acc_sum.zero();
+
+    int it = 0;
+    for (int64_t t = t0 + 4 * wave; t < t1; t += 4 * nw) {
+        const bool valid = t + grp < t1;
+        const uint32_t tu = (uint32_t)(valid ? t + grp : t);
+        const int k = gl < ne ? (int)((tu / my_w) % (uint32_t)(my_e + 1)) : 0;
+        const int delta = gl < ne ? (glynn ? 2 * k - my_e : k) : 0;
+        int ksum = k;
+        double bin = gl < ne ? btab[gl * (H + 1) + k] : 1.0;
+        // over the term's 16 lanes (lane bits 0, 1, 4, 5): every lane receives the total
+        ksum += __shfl_xor(ksum, 1, WH_WAVE);
+        bin *= __shfl_xor(bin, 1, WH_WAVE);
+        ksum += __shfl_xor(ksum, 2, WH_WAVE);
+        bin *= __shfl_xor(bin, 2, WH_WAVE);
+        ksum += __shfl_xor(ksum, 16, WH_WAVE);
+        bin *= __shfl_xor(bin, 16, WH_WAVE);
+        ksum += __shfl_xor(ksum, 32, WH_WAVE);
+        bin *= __shfl_xor(bin, 32, WH_WAVE);  // exact: a product of integers below 2^53
+        const int delta0 = __shfl(delta, gbase, WH_WAVE);
+        double prefac = (((h - ksum) & 1) ? -1.0 : 1.0) * bin;
+        if (glynn && delta0 == 0) prefac *= 0.5;
+        if (!valid) prefac = 0.0;
+        double sgnJ[3], sgnI[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double dj = (double)__shfl(delta, dlane(srcJ[c]), WH_WAVE);
+            const double di = (double)__shfl(delta, dlane(srcI[c]), WH_WAVE);
+            sgnJ[c] = 4 * c + x < n ? dj : 0.0;
+            sgnI[c] = 4 * c + y < n ? di : 0.0;
+        }
+        const double dl = (double)__shfl(delta, dlane(src_l), WH_WAVE);
+
+        cplx* pt = slots + (it * 4 + grp) * 2 * PTN;
+        cplx* lv = pt + PTN;
+        E acc[3][3];
+#pragma unroll
+        for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 3; ++bj) acc[bi][bj] = escale(Ms[(4 * bi + y) * S + 4 * bj + x], sgnJ[bj]);
+        E zprev[3], xdI[3];
+        if (LOOP) {
+#pragma unroll
+            for (int bi = 0; bi < 3; ++bi) {
+                zprev[bi] = Dv[4 * bi + y];
+                xdI[bi] = escale(XD0[4 * bi + y], sgnI[bi]);
+            }
+        }
+        auto store = [&]() {
+            wave_lds_sync();  // the term's reads of the previous power are done
+#pragma unroll
+            for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 3; ++bj) Pw[(4 * bi + y) * S + 4 * bj + x] = acc[bi][bj];
+            wave_lds_sync();  // the whole power is stored before any lane reads it
+        };
+        auto tdot = [&]() {  // lane part of <acc, Pw^T>
+            E sd;
+            ezero(sd);
+#pragma unroll
+            for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 3; ++bj) emac(sd, acc[bi][bj], Pw[(4 * bj + x) * S + 4 * bi + y]);
+            return sd;
+        };
+        store();
+        const int Ah = (h + 1) / 2;
+        for (int a = 1; a <= Ah; ++a) {
+            E qo, qe;
+            if (a == 1) {
+                ezero(qo);
+                if (x == y) {
+#pragma unroll
+                    for (int bb = 0; bb < 3; ++bb) qo = eadd(qo, acc[bb][bb]);
+                }
+                qe = tdot();
+            } else {
+                // acc = (Pw M) diag(delta) on the matrix cores, the four terms at once
+                double X[3][3], Y[3][3], Z[3][3];
+#pragma unroll
+                for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 3; ++bj) X[bi][bj] = Y[bi][bj] = Z[bi][bj] = 0.0;
+#pragma unroll
+                for (int kc = 0; kc < 3; ++kc) {
+                    E av[3], bv[3];
+#pragma unroll
+                    for (int bi = 0; bi < 3; ++bi) av[bi] = Pw[(4 * bi + x) * S + 4 * kc + y];
+#pragma unroll
+                    for (int bj = 0; bj < 3; ++bj) bv[bj] = Ms[(4 * kc + y) * S + 4 * bj + x];
+                    if constexpr (REAL) {
+#pragma unroll
+                        for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 3; ++bj) X[bi][bj] = r4m_mfma(av[bi], bv[bj], X[bi][bj]);
+                    } else {
+                        double as[3], bs[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            as[c] = av[c].re + av[c].im;
+                            bs[c] = bv[c].re + bv[c].im;
+                        }
+#pragma unroll
+                        for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 3; ++bj) {
+                                X[bi][bj] = r4m_mfma(av[bi].re, bv[bj].re, X[bi][bj]);
+                                Y[bi][bj] = r4m_mfma(av[bi].im, bv[bj].im, Y[bi][bj]);
+                                Z[bi][bj] = r4m_mfma(as[bi], bs[bj], Z[bi][bj]);
+                            }
+                    }
+                }
+#pragma unroll
+                for (int bi = 0; bi < 3; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 3; ++bj) {
+                        if constexpr (REAL) {
+                            acc[bi][bj] = escale(eget(cmk(X[bi][bj], 0.0), (E*)nullptr), sgnJ[bj]);
+                        } else {
+                            const double xx = X[bi][bj], yy = Y[bi][bj];
+                            acc[bi][bj] = escale(eget(cmk(xx - yy, Z[bi][bj] - (xx + yy)), (E*)nullptr), sgnJ[bj]);
+                        }
+                    }
+                qo = tdot();
+                store();
+                qe = tdot();
+            }
+            E de, dodd;
+            ezero(de);
+            ezero(dodd);
+            E znew[3];
+            if (LOOP) {
+                // y = XD_d B^a (the lane's columns 4 bj + x, over its term's rows), z = B^a D (its
+                // rows 4 bi + y, over the columns); the dots pair them on the lanes with x == y
+                E yv[3];
+#pragma unroll
+                for (int bj = 0; bj < 3; ++bj) {
+                    ezero(yv[bj]);
+#pragma unroll
+                    for (int bi = 0; bi < 3; ++bi) emac(yv[bj], xdI[bi], acc[bi][bj]);
+                    yv[bj] = ycol_e(yv[bj]);
+                }
+#pragma unroll
+                for (int bi = 0; bi < 3; ++bi) {
+                    ezero(znew[bi]);
+#pragma unroll
+                    for (int bj = 0; bj < 3; ++bj) emac(znew[bi], acc[bi][bj], Dv[4 * bj + x]);
+                    znew[bi] = xrow_e(znew[bi]);
+                }
+                if (x == y) {
+#pragma unroll
+                    for (int bb = 0; bb < 3; ++bb) {
+                        emac(de, yv[bb], zprev[bb]);
+                        emac(dodd, yv[bb], znew[bb]);
+                    }
+                }
+            }
+            const cplx qoc = etoc(qo), qec = etoc(qe), dec = etoc(de), doc = etoc(dodd);
+            const double v8[8] = {qoc.re, qoc.im, qec.re, qec.im, dec.re, dec.im, doc.re, doc.im};
+            const double sum8 = term16_sum8(v8, lane);
+            const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
+            if ((lane & 1) == 0) {  // the lane with bit 0 clear of each value's pair writes it
+                const int kv = ((lane >> 1) & 1) | (((lane >> 4) & 3) << 1);
+                double* dst = nullptr;
+                if (kv < 2) dst = &pt[ko].re;
+                else if (kv < 4) dst = &pt[ke].re;  // ke <= h + 1 < PTN
+                else if (LOOP && kv < 6 && 2 * a <= h) dst = &lv[2 * a].re;
+                else if (LOOP && kv >= 6 && 2 * a + 1 <= h) dst = &lv[2 * a + 1].re;
+                if (dst) dst[kv & 1] = sum8;
+            }
+            if (LOOP) {
+#pragma unroll
+                for (int bi = 0; bi < 3; ++bi) zprev[bi] = znew[bi];
+            }
+        }
+        if (LOOP) {
+            // v_1 = XD_d . D (lane gl < n of the term holds index gl)
+            E part;
+            ezero(part);
+            if (gl < n) emac(part, escale(XD0[gl], dl), Dv[gl]);
+            const cplx v1 = etoc(term16_e(part));
+            if (gl == 0) lv[1] = v1;
+        }
+        if (gl == 0) pt[0] = cmk(prefac, 0.0);
+        ++it;
+        if (it == NI || t + 4 * nw >= t1) {
+            wave_lds_sync();  // every term's slots as their lanes stored them
+            if (lane < 4 * it) {  // lane j: slot j (iteration j / 4, term j % 4)
+                cplx* ptj = slots + lane * 2 * PTN;
+                const double pf = ptj[0].re;
+                if (pf != 0.0) {
+                    const E top = coeff_top_lds<E, H, LOOP>(ptj, ptj + PTN, h);
+                    acc_sum.add(cmulr(etoc(top), pf));
+                }
+            }
+            wave_lds_sync();  // the slots are free again
+            it = 0;
+        }
+    }
+
+    double4* lsum = reinterpret_cast<double4*>(slots);
+    if (lane < kR4Slots) lsum[lane] = make_double4(acc_sum.sr, acc_sum.cr, acc_sum.si, acc_sum.ci);
+    wave_lds_sync();
+    if (lane == 0) {
+        csum sm;
+        sm.zero();
+        for (int j = 0; j < kR4Slots; ++j) {
+            double4 pq = lsum[j];
+            sm.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        wres[wave] = make_double4(sm.sr, sm.cr, sm.si, sm.ci);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        csum sm;
+        sm.zero();
+        for (int w = 0; w < nw; ++w) {
+            double4 pq = wres[w];
+            sm.add_pair(pq.x, pq.y, pq.z, pq.w);
+        }
+        partials[b] = make_double4(sm.sr, sm.cr, sm.si, sm.ci);
+    }
+}
+
 }  // namespace
 
 template <int T, bool REAL, bool LOOP, bool HALF>
@@ -1210,8 +1558,13 @@ wh_sieve_rep(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restr
     rep_body<T, REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
 }
 
+// Two waves per SIMD: the matrix-core form's complex loop instances otherwise take 266 registers
+// per lane (one wave per SIMD); the LDS holds two workgroups of four waves per CU.
+#ifndef WH_REP4_WAVES_PER_EU
+#define WH_REP4_WAVES_PER_EU 2
+#endif
 template <bool REAL, bool LOOP, int H>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WH_REP4_WAVES_PER_EU)))
 wh_sieve_rep4(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
               double4* __restrict__ partials, int64_t block_offset) {
     extern __shared__ __align__(32) unsigned char wh_smem[];
@@ -1221,7 +1574,11 @@ wh_sieve_rep4(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __rest
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].chunk_base <= b) lo = mid; else hi = mid - 1;
     }
+#if WH_REP4_MFMA
+    rep4m_body<REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
+#else
     rep4_body<REAL, LOOP, H>(jobs[lo], pool, partials, b, wh_smem);
+#endif
 }
 
 template <bool REAL, bool LOOP, int H>
