@@ -17,7 +17,8 @@ Warm-up: the W steps of a warm-up stream (seed 142) run batched and served first
 
 The same JSON line also carries:
   * batched: the batched capacity C (one plan per step, inputs already in HBM) and served / C;
-  * serving_half_load: latencies of the first two steps served at 0.5 C;
+  * serving_half_load / serving_0.9_load: latencies of the first two steps served at 0.5 C and
+    at 0.9 C;
   * warmup_step_served_closed_loop / warmup_step_batched: one warm-up step submitted to the
     server at once, and the same step as one resident batch;
   * per_call: the same K R requests through the drop-in ``hafnian()`` one call at a time
@@ -619,9 +620,10 @@ def main():
     batched = {"value": capacity, "unit": "hafnians/s", "ms_per_step": bwall / K * 1e3,
                "served_over_batched": value / capacity,
                "served_max_rel_diff_vs_batched": max_rel(served_vals, bvals)}
-    # latency at half load on the first two steps
+    # latency at half and at 0.9 of the capacity on the first two steps
     half = reqs[:2 * R]
     _, stats_half, _ = serve_stream(half, 0.5 * capacity, Server, seed=143)
+    _, stats_09, _ = serve_stream(half, 0.9 * capacity, Server, seed=144)
 
     roof = dominant_kernel(reqs[:R], B, torch, stream)
     if roof:
@@ -686,6 +688,7 @@ def main():
         "p99_ms": stats.get("p99_ms"),
         "serving": stats,
         "serving_half_load_first_%d" % len(half): stats_half,
+        "serving_0.9_load_first_%d" % len(half): stats_09,
         "warmup_step_served_closed_loop": served_cap,
         "warmup_step_batched": R / wstep,
         "batched": batched,

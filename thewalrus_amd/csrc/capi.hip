@@ -378,14 +378,16 @@ int pm1_waves(int T, bool real, int kind) {
         if (b > lds_cap) break;
         int per_cu = (int)(lds_cap / b) * nw;
         if (per_cu > 16) per_cu = 16;
-        if (per_cu > best_w) { best_w = per_cu; best = nw; }
+        // ties go to the larger workgroup: the same waves per CU with fewer staging copies of
+        // the matrix (the real symmetric instance at T = 3: 33.1 -> 30.5 ms on 4096 n = 24)
+        if (per_cu >= best_w) { best_w = per_cu; best = nw; }
     }
     return best;
 }
 
 // Workgroups per request the chunk size aims at, per kernel class (wh_tune_set; 0 = the
 // defaults below): [0][T][loop] all-ones kernels by tile side class, [1][0][loop] general kernel,
-// [2][T][loop] repeated-row kernel (T = 1, 2).
+// [2][T][loop] repeated-row kernel (tile sides T = 1..3).
 std::atomic<int64_t> g_tune_blocks[3][kPm1MaxT + 1][2];
 // Waves per workgroup per kernel class (wh_tune_set_waves; 0 = the LDS-based choice), same indexing.
 std::atomic<int> g_tune_waves[3][kPm1MaxT + 1][2];
