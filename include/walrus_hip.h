@@ -340,6 +340,13 @@ int wh_tune_set_tile(int kind, int T, int loop, int tile);
 /* The tile entry, or -1 for an entry out of range. */
 int wh_tune_get_tile(int kind, int T, int loop);
 
+/* ---- Failure injection.  No counterpart in the reference (its kernels cannot fail); for tests of
+ * the error paths of plans, batches, per-call entry points and the serving engine.  The next
+ * `count` batch launches fail with WH_EDEVICE ("... unspecified launch failure") before any kernel
+ * is launched, so no device state is touched; the serving engine fails that batch's requests and
+ * keeps serving.  count < 0 reads the number left; returns the previous number. */
+int wh_debug_fail_launches(int count);
+
 /* How plans and the serving engine launch a batch's complex all-ones requests with n in [26, 48]
  * (the symmetric matrix-core kernel's tile sides 4..6): 0 (default) = one launch per kernel
  * class, then the fold; 1 = one persistent launch for all of them (workgroups take the classes'

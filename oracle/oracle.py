@@ -27,7 +27,8 @@ def lib():
     """Load (building first if needed) the oracle shared library."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libwalrus_oracle.so")
+        # WALRUS_ORACLE_LIB may name another build (the sanitizer build of ``make sanitize``)
+        path = os.environ.get("WALRUS_ORACLE_LIB") or os.path.join(_HERE, "libwalrus_oracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -425,7 +426,8 @@ def cpu_timing_lib():
     global _CT
     if _CT is None:
         isa = "x86-64-v4" if _host_has_avx512() else "x86-64-v3"
-        path = os.path.join(_HERE, "libcpu_timing_%s.so" % ("v4" if isa == "x86-64-v4" else "v3"))
+        path = os.environ.get("WALRUS_CPU_TIMING_LIB") or \
+            os.path.join(_HERE, "libcpu_timing_%s.so" % ("v4" if isa == "x86-64-v4" else "v3"))
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)

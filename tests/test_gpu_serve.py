@@ -129,3 +129,40 @@ def test_close_runs_queued_requests():
     assert srv.stats.launched == 41 and srv.stats.batches >= 2
     with pytest.raises(RuntimeError):
         srv.submit(small[0])
+
+
+def test_injected_launch_failures_fail_the_batch_and_the_server_keeps_serving():
+    """wh_debug_fail_launches makes the next batch launches fail before any kernel runs: a
+    resident batch and a per-call hafnian raise RuntimeError and then run again with the same
+    values; in the server the requests of the failed batch get RuntimeError and the later ones
+    their values, from the same server."""
+    from thewalrus_amd import _native
+    from thewalrus_amd import batch as B
+
+    L = _native.load()
+    rng = np.random.default_rng(23)
+    mats = [random_symmetric(rng, 2 * int(rng.integers(4, 12)), scale=0.3) for _ in range(12)]
+    want = [tw.hafnian(A) for A in mats]
+    try:
+        db = B.DeviceBatch([B.prepare(A) for A in mats])
+        assert L.wh_debug_fail_launches(1) == 0
+        with pytest.raises(RuntimeError, match="launch failure"):
+            db.run()
+        assert L.wh_debug_fail_launches(-1) == 0
+        db.run()
+        assert np.array_equal(db.results(), np.array(want))
+        db.close()
+        L.wh_debug_fail_launches(1)
+        with pytest.raises(RuntimeError):
+            tw.hafnian(mats[0])
+        assert tw.hafnian(mats[0]) == want[0]
+
+        with Server(slots=1, slot_pool_mb=2) as srv:
+            L.wh_debug_fail_launches(1)
+            first = srv.submit(mats[0])
+            with pytest.raises(RuntimeError, match="launch failure"):
+                first.result(timeout=60)
+            futs = [srv.submit(A) for A in mats]
+            assert [f.result(timeout=60) for f in futs] == want
+    finally:
+        L.wh_debug_fail_launches(0)

@@ -334,3 +334,24 @@ def test_loop_hafnian_batch_gamma_rows_vs_kan():
     for k in range(3):
         for q in range(4):
             assert relerr(H[k, q], O.kan(A, [1, 0, 2, q], mu=D[k], loop=True)) < 1e-14
+
+
+@pytest.mark.parametrize("n", [6, 8, 12, 14])
+@pytest.mark.parametrize("loop", [False, True])
+def test_timed_cpu_baseline_matches_the_oracle(n, loop):
+    """The timed CPU baseline (cpu_timing.cpp: the reference's per-term products, FMA contraction
+    on) gives the oracle's hafnian to rounding, over all terms at once and over a split range of
+    terms (the bounded samples bench.py times); J_n gives (n - 1)!!."""
+    rng = np.random.default_rng(300 + n + loop)
+    A = random_symmetric(rng, n, scale=1 / np.sqrt(n))
+    Ax, Dx = O.ct_prepare(A, loop)
+    got = O.ct_sieve_many([Ax, Ax], None if Dx is None else [Dx, Dx], nthreads=2)
+    want = O.hafnian_sieve(A, loop=loop, precision="dd")
+    assert relerr(got[0], want) < 1e-11 and got[0] == got[1]
+    steps = 1 << (n // 2 - 1)
+    half = steps // 2
+    split = O.ct_sieve(Ax, Dx, 0, half, nthreads=2) + O.ct_sieve(Ax, Dx, half, steps, nthreads=2)
+    assert relerr(split * 2.0 ** -(n // 2 - 1), want) < 1e-11
+    J = np.ones((n, n))
+    Jx, JD = O.ct_prepare(J, False)
+    assert O.ct_sieve_many([Jx], None)[0] == math.prod(range(n - 1, 0, -2))

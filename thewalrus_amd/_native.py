@@ -76,6 +76,7 @@ EXPORTS = [
     "wh_tune_set_tile",
     "wh_tune_get_tile",
     "wh_tune_persistent",
+    "wh_debug_fail_launches",
     "wh_loop_hafnian_batch",
     "wh_loop_hafnian_batch_gamma",
     "wh_montrealer",
@@ -133,6 +134,7 @@ def load(path=None):
         L.wh_tune_persistent.argtypes = [ctypes.c_int]
         L.wh_tune_set_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_tune_set_tile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.wh_debug_fail_launches.argtypes = [ctypes.c_int]
         L.wh_tune_get_waves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_tune_get_tile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.wh_batch_create.argtypes = [ctypes.POINTER(WhRequest), ctypes.c_int64, ctypes.c_int64,

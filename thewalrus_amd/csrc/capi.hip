@@ -572,8 +572,15 @@ int device_cus(int dev) {
 bool mixed_group(const GroupKey& k) { return k.kind == KIND_PM1S && !k.real && k.T >= 4 && k.T <= 6; }
 
 // Enqueues the sieve kernels of every group and the fold.
+// Launches the next wh_debug_fail_launches() count of batch launches refuse (tests of the error
+// paths: nothing is launched, so no device state is touched).
+std::atomic<int> g_fail_launches{0};
+
 hipError_t run_layout(const Layout& L, const wh_job_dev* d_jobs, const void* pool, double4* partials,
                       void* out, hipStream_t s, unsigned* counters = nullptr) {
+    for (int f = g_fail_launches.load(std::memory_order_relaxed); f > 0;
+         f = g_fail_launches.load(std::memory_order_relaxed))
+        if (g_fail_launches.compare_exchange_weak(f, f - 1, std::memory_order_relaxed)) return hipErrorLaunchFailure;
     std::vector<wh_mixed_group> mixed;
     const int mode = g_persistent.load(std::memory_order_relaxed);
     if (counters && mode > 0) {
@@ -929,6 +936,11 @@ int wh_tune_get_tile(int kind, int T, int loop) {
         (kind == 2 && (T < 1 || T > kRepMaxT)))
         return -1;
     return g_tune_tile[kind][T][loop].load(std::memory_order_relaxed);
+}
+
+int wh_debug_fail_launches(int count) {
+    if (count < 0) return g_fail_launches.load(std::memory_order_relaxed);
+    return g_fail_launches.exchange(count, std::memory_order_relaxed);
 }
 
 int wh_tune_persistent(int mode) {
