@@ -127,9 +127,10 @@ def _class_workload(kind, T, loop, rng):
         return [B.prepare(sym(n), loop=bool(loop)) for _ in range(count)]
     if kind == 2:
         # repeated-row kernel, 32 photons: 4 modes (2E <= 8, side 1), 8 modes (2E in 10..12,
-        # side 3: four terms per wave) or 16 modes (2E in 14..16, side 2; fewer requests, as each
-        # has tens of thousands of terms)
-        modes, count = {1: (4, 256), 3: (8, 256), 2: (16, 32)}[T]
+        # side 3: four terms per wave; 1024 requests, the repeated-row benchmark's batch, whose
+        # hundreds of workgroups per CU change the best waves count) or 16 modes (2E in 14..16,
+        # side 2; fewer requests, as each has tens of thousands of terms)
+        modes, count = {1: (4, 256), 3: (8, 1024), 2: (16, 32)}[T]
         preps = []
         while len(preps) < count:
             rpt = rng.multinomial(32, [1 / modes] * modes)
