@@ -275,9 +275,9 @@ def configs_run(torch, tw, B, stream, with_cpu=True):
 
 
 def next_rows_run(with_cpu):
-    """The sampler's batched loop hafnian and the montrealer: device time per call (host
-    buffers in and out, as the drop-ins take them) and, with with_cpu, the CPU oracle on the
-    same inputs."""
+    """The next rows: the samplers' batched loop hafnians, the montrealer, hafnian_batched (the
+    Hermite recursion) and a probability tensor: device time per call (host buffers in and out,
+    as the drop-ins take them) and, with with_cpu, the CPU oracle on the same inputs."""
     from thewalrus_amd import lmtl, mtl
     from thewalrus_amd.loop_hafnian_batch import loop_hafnian_batch
     from thewalrus_amd.loop_hafnian_batch_gamma import loop_hafnian_batch_gamma
@@ -326,6 +326,30 @@ def next_rows_run(with_cpu):
         if with_cpu and nm == 16:
             row["cpu_oracle_ms"] = per_call_ms(lambda: O.mtl(S), 1)
         out["mtl_n%d" % nm] = row
+
+    # hafnian_batched: the Hermite recursion's whole tensor, 6 modes, cutoff 10 (10^6 entries)
+    from thewalrus_amd._hermite_multidimensional import hafnian_batched
+
+    nh = 6
+    Xh = rng.normal(size=(nh, nh)) + 1j * rng.normal(size=(nh, nh))
+    Ah = (Xh + Xh.T) * 0.15
+    muh = 0.3 * (rng.normal(size=nh) + 1j * rng.normal(size=nh))
+    row = {"workload": "hafnian_batched, %d modes, cutoff 10, loop weights (the whole tensor)" % nh,
+           "gpu_ms": per_call_ms(lambda: hafnian_batched(Ah, 10, mu=muh), 3)}
+    if with_cpu:
+        row["cpu_oracle_ms"] = per_call_ms(lambda: O.hafnian_batched(Ah, 10, mu=muh), 1)
+    out["hafnian_batched"] = row
+
+    # probabilities of a 4-mode Gaussian state up to 6 photons per mode (1296 density-matrix
+    # diagonal elements: repeated-row loop hafnians, one device batch)
+    from thewalrus_amd import quantum as Q
+
+    nq = 4
+    Sg = rng.normal(size=(2 * nq, 2 * nq))
+    cov = np.identity(2 * nq) + 0.2 * (Sg @ Sg.T) / (2 * nq)
+    muq = 0.3 * rng.normal(size=2 * nq)
+    out["probabilities"] = {"workload": "probabilities, %d modes, cutoff 6, displaced mixed state" % nq,
+                            "gpu_ms": per_call_ms(lambda: Q.probabilities(muq, cov, 6), 3)}
     return out
 
 
