@@ -73,7 +73,7 @@ def per_call(reqs, tw):
     return len(reqs) / (time.perf_counter() - t0), vals
 
 
-def serve_closed(reqs, Server, reps=3):
+def serve_closed(reqs, Server, reps=7):
     """Every request submitted at once to a server, after one untimed pass (first launch of
     each kernel class), ``reps`` times: (median hafnians/s from the first submission to the
     last result, the values of the last pass)."""
