@@ -260,7 +260,7 @@ GroupKey classify_auto(const wh_job_dev& j, int options);
 
 // The LDS bytes of a workgroup of nw waves for an all-ones or repeated-row group.
 size_t group_lds_bytes(const GroupKey& g, int nw) {
-    if (g.kind == KIND_PM1S) return wh_pm1s_lds_bytes(g.T, nw);
+    if (g.kind == KIND_PM1S) return wh_pm1s_lds_bytes(g.T, nw, g.real != 0);
     if (g.kind == KIND_PM1M) return wh_pm1m_lds_bytes(g.T, nw);
     if (g.kind == KIND_PM1) return wh_pm1_lds_bytes(g.T, g.real != 0, nw);
     if (g.kind == KIND_REP) return wh_rep_lds_bytes(g.T, g.real != 0, nw, g.H);
@@ -372,7 +372,7 @@ int pm1_waves(int T, bool real, int kind) {
     const size_t lds_cap = 160 * 1024;
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
-        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw)
+        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real)
                    : kind == KIND_PM1M ? wh_pm1m_lds_bytes(T, nw)
                                        : wh_pm1_lds_bytes(T, real, nw);
         if (b > lds_cap) break;

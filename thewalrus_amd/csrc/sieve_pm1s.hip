@@ -81,6 +81,18 @@ __device__ __forceinline__ cplx ldc(const cplx* p) {
     if constexpr (RE) return cmk(p->re, 0.0);
     else return *p;
 }
+// Entry i of a matrix in LDS (X A, the powers V): complex, or for the real instance its real
+// part only, the matrices then laid out as doubles (half the LDS: more workgroups per CU)
+template <bool RE>
+__device__ __forceinline__ cplx mget(const cplx* m, int i) {
+    if constexpr (RE) return cmk(reinterpret_cast<const double*>(m)[i], 0.0);
+    else return m[i];
+}
+template <bool RE>
+__device__ __forceinline__ void mput(cplx* m, int i, cplx v) {
+    if constexpr (RE) reinterpret_cast<double*>(m)[i] = v.re;
+    else m[i] = v;
+}
 
 template <int T> struct GeoS {
     static constexpr int NP = 8 * T;
@@ -140,7 +152,7 @@ __device__ __forceinline__ void sym_tdot_load(TdotVals<T>& X, const cplx* Vb, co
     for (int p = 0; p < T; ++p) {
         const int rs = swp(L.rd(p), h);
 #pragma unroll
-        for (int q = p; q < T; ++q) X.x[g++] = ldc<RE>(&Vb[swp(L.cd(q), h) * S + rs]);
+        for (int q = p; q < T; ++q) X.x[g++] = mget<RE>(Vb, swp(L.cd(q), h) * S + rs);
     }
 }
 template <int T>
@@ -173,13 +185,8 @@ __device__ __forceinline__ void sym_store(cplx* X, const double (&aR)[T][T], con
 #pragma unroll
         for (int q = p; q < T; ++q) {
             const cplx v = cflip(cmk(aR[p][q], aI[p][q]), (fm >> (T + q)) & 1u);
-            if constexpr (RE) {  // the real halves only
-                X[L.rd(p) * S + L.cd(q)].re = v.re;
-                if (p < q) X[L.cd(q) * S + L.rd(p)].re = v.re;
-            } else {
-                X[L.rd(p) * S + L.cd(q)] = v;
-                if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
-            }
+            mput<RE>(X, L.rd(p) * S + L.cd(q), v);
+            if (p < q) mput<RE>(X, L.cd(q) * S + L.rd(p), v);
         }
 }
 
@@ -222,10 +229,10 @@ __device__ __forceinline__ void epis_load_row(int r, EpiS<T>& E, const LaneMapS&
     constexpr int S = GeoS<T>::S;
     const int rs = swp(L.rd(r), h);
 #pragma unroll
-    for (int q = r; q < T; ++q) E.tx[q] = ldc<RE>(&Vb[swp(L.cd(q), h) * S + rs]);
+    for (int q = r; q < T; ++q) E.tx[q] = mget<RE>(Vb, swp(L.cd(q), h) * S + rs);
     if (LOOP) {  // A[rd(r)][cd(q)] = (X A)[swap rd(r)][cd(q)]
 #pragma unroll
-        for (int q = 0; q < T; ++q) E.av[q] = ldc<RE>(&Ms[rs * S + L.cd(q)]);
+        for (int q = 0; q < T; ++q) E.av[q] = mget<RE>(Ms, rs * S + L.cd(q));
     }
 }
 
@@ -377,13 +384,8 @@ __device__ __forceinline__ void bnd_step(int gi, BndS<T>& Bd, cplx* X, double (&
         Bd.sd = Bd.so = cmk(0.0, 0.0);
     }
     const cplx v = cflip(r, (fm >> (T + q)) & 1u);
-    if constexpr (RE) {
-        X[L.rd(p) * S + L.cd(q)].re = v.re;
-        if (p < q) X[L.cd(q) * S + L.rd(p)].re = v.re;
-    } else {
-        X[L.rd(p) * S + L.cd(q)] = v;
-        if (p < q) X[L.cd(q) * S + L.rd(p)] = v;
-    }
+    mput<RE>(X, L.rd(p) * S + L.cd(q), v);
+    if (p < q) mput<RE>(X, L.cd(q) * S + L.rd(p), v);
 }
 
 // With the fused boundary (pm1s_fused) the product also overwrites Vb with V_a and returns the
@@ -413,13 +415,13 @@ __device__ __forceinline__ cplx sym_product_epi(cplx* Vb, const cplx* __restrict
         }
     EpiS<T> E;
     epis_init<T, LOOP, RE>(E, L, zprev, fm, h);
-    const cplx* pa = Vb + (4 * L.br + L.x) * S + L.y;
-    const cplx* pb = Ms + L.y * S + 4 * L.bc + L.x;
+    const int ia = (4 * L.br + L.x) * S + L.y;
+    const int ib = L.y * S + 4 * L.bc + L.x;
     cplx avn[T], bqn[T];
 #pragma unroll
-    for (int p = 0; p < T; ++p) avn[p] = RE ? cmk(pa[8 * p * S].re, 0.0) : pa[8 * p * S];
+    for (int p = 0; p < T; ++p) avn[p] = mget<RE>(Vb, ia + 8 * p * S);
 #pragma unroll
-    for (int q = 0; q < T; ++q) bqn[q] = RE ? cmk(pb[8 * q].re, 0.0) : pb[8 * q];
+    for (int q = 0; q < T; ++q) bqn[q] = mget<RE>(Ms, ib + 8 * q);
     epis_load_row<T, LOOP, RE>(0, E, L, Vb, Ms, h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
@@ -431,10 +433,10 @@ __device__ __forceinline__ cplx sym_product_epi(cplx* Vb, const cplx* __restrict
         if (kc + 1 < NK) {
 #pragma unroll
             for (int p = 0; p < T; ++p)
-                avn[p] = RE ? cmk(pa[8 * p * S + 4 * (kc + 1)].re, 0.0) : pa[8 * p * S + 4 * (kc + 1)];
+                avn[p] = mget<RE>(Vb, ia + 8 * p * S + 4 * (kc + 1));
 #pragma unroll
             for (int q = 0; q < T; ++q)
-                bqn[q] = RE ? cmk(pb[4 * (kc + 1) * S + 8 * q].re, 0.0) : pb[4 * (kc + 1) * S + 8 * q];
+                bqn[q] = mget<RE>(Ms, ib + 4 * (kc + 1) * S + 8 * q);
         }
         if ((kc & 1) == 1 && kc + 1 < NK) epis_load_row<T, LOOP, RE>((kc + 1) >> 1, E, L, Vb, Ms, h);
         if (FB && kc == NK - 1) {
@@ -570,12 +572,13 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     const int n = 2 * J.E;
     const int h = J.E;
 
-    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave, D, per wave NB term slots (pt, lv)
+    // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave (doubles for the real instance), D,
+    // per wave NB term slots (pt, lv)
     // and z rows x 2
     constexpr int PTN = GeoS<T>::PTN;
     constexpr int NB = GeoS<T>::NB;
     cplx* Ms = reinterpret_cast<cplx*>(smem);
-    const size_t mat = ((size_t)NP * S * sizeof(cplx) + 31) / 32 * 32 / sizeof(cplx);
+    const size_t mat = ((size_t)NP * S * (RE ? sizeof(double) : sizeof(cplx)) + 31) / 32 * 32 / sizeof(cplx);
     cplx* Pw = Ms + mat + wave * mat;
     cplx* Dv = Ms + (1 + nw) * mat;
     constexpr int SMALL = NB * 2 * PTN + 2 * NP;
@@ -589,7 +592,7 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
         int r = e / NP, c = e - r * NP;
         cplx v = cmk(0.0, 0.0);
         if (r < n && c < n) v = A[(int64_t)swp(r, h) * n + c];
-        Ms[r * S + c] = v;
+        mput<RE>(Ms, r * S + c, v);
     }
     if (LOOP) {
         const cplx* Dg = pool + J.d_off;
@@ -625,14 +628,14 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
         for (int p = 0; p < T; ++p)
 #pragma unroll
             for (int q = p; q < T; ++q) {
-                const cplx m = cflip(ldc<RE>(&Ms[swp(L.rd(p), h) * S + L.cd(q)]), (fm >> p) & 1u);
+                const cplx m = cflip(mget<RE>(Ms, swp(L.rd(p), h) * S + L.cd(q)), (fm >> p) & 1u);
                 aR[p][q] = m.re;
                 aI[p][q] = RE ? 0.0 : m.im;
             }
         sym_store<T, RE>(Pw, aR, aI, L, fm);
         // p_1 = tr(B) = sum_e d_e A[e][swap e] (lane partials, summed with power 1's results)
         cplx qo = cmk(0.0, 0.0);
-        if (lane < n) qo = cmulr(ldc<RE>(&Ms[swp(lane, h) * S + swp(lane, h)]), sgn(tu, lane, h));
+        if (lane < n) qo = cmulr(mget<RE>(Ms, swp(lane, h) * S + swp(lane, h)), sgn(tu, lane, h));
         if (LOOP) {
             for (int e = lane; e < NP; e += WH_WAVE) Zb[e] = Dv[e];  // z_0 = D
         }
@@ -803,10 +806,10 @@ wh_sieve_pm1s_persistent(const wh_job_dev* __restrict__ jobs, const cplx* __rest
 
 __global__ void wh_clear_counters(unsigned* __restrict__ counters) { counters[threadIdx.x] = 0u; }
 
-size_t wh_pm1s_lds_bytes(int T, int nw) {
+size_t wh_pm1s_lds_bytes(int T, int nw, bool real) {
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1S_PAD;
     size_t PTN = 4 * (size_t)T + 4, NB = T >= 6 ? 1 : T == 5 ? 7 : 8;
-    size_t mat = (NP * S * sizeof(cplx) + 31) / 32 * 32;
+    size_t mat = (NP * S * (real ? sizeof(double) : sizeof(cplx)) + 31) / 32 * 32;
     size_t small = (NB * 2 * PTN + 2 * NP) * sizeof(cplx);
     return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
@@ -814,7 +817,7 @@ size_t wh_pm1s_lds_bytes(int T, int nw) {
 template <int T, bool LOOP, bool RE>
 static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
-    size_t lds = wh_pm1s_lds_bytes(T, nw);
+    size_t lds = wh_pm1s_lds_bytes(T, nw, RE);
     static wh_lds_limit lim;
     hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lim);
     if (err != hipSuccess) return err;
@@ -834,7 +837,7 @@ hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, 
     for (int k = 0; k < ngroups; ++k) {
         items.g[k] = groups[k];
         if (groups[k].T < 1 || groups[k].T > 6 || groups[k].nw < 1 || groups[k].nw > 4) return hipErrorInvalidValue;
-        lds = std::max(lds, wh_pm1s_lds_bytes(groups[k].T, groups[k].nw));
+        lds = std::max(lds, wh_pm1s_lds_bytes(groups[k].T, groups[k].nw, false));
     }
     static wh_lds_limit lim;
     hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s_persistent, lds, lim);
