@@ -260,7 +260,7 @@ GroupKey classify_auto(const wh_job_dev& j, int options);
 
 // The LDS bytes of a workgroup of nw waves for an all-ones or repeated-row group.
 size_t group_lds_bytes(const GroupKey& g, int nw) {
-    if (g.kind == KIND_PM1S) return wh_pm1s_lds_bytes(g.T, nw, g.real != 0);
+    if (g.kind == KIND_PM1S) return wh_pm1s_lds_bytes(g.T, nw, g.real != 0, g.loop != 0);
     if (g.kind == KIND_PM1M) return wh_pm1m_lds_bytes(g.T, nw);
     if (g.kind == KIND_PM1) return wh_pm1_lds_bytes(g.T, g.real != 0, nw);
     if (g.kind == KIND_REP) return wh_rep_lds_bytes(g.T, g.real != 0, nw, g.H);
@@ -368,11 +368,11 @@ int rep_waves(int T, bool real, int H) {
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
 // under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
-int pm1_waves(int T, bool real, int kind) {
+int pm1_waves(int T, bool real, bool loop, int kind) {
     const size_t lds_cap = 160 * 1024;
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
-        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real)
+        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real, loop)
                    : kind == KIND_PM1M ? wh_pm1m_lds_bytes(T, nw)
                                        : wh_pm1_lds_bytes(T, real, nw);
         if (b > lds_cap) break;
@@ -521,7 +521,7 @@ int build_layout(const wh_request* reqs, int64_t n, int64_t chunk_hint, Layout& 
             const int tw = tuned_waves(g);  // the tuning table's choice, where it fits the LDS
             gr.nw = g.nw                                                            ? g.nw
                     : (tw >= 1 && tw <= 4 && group_lds_bytes(g, tw) <= 160 * 1024) ? tw
-                    : all_ones_kind(g.kind)                                         ? pm1_waves(g.T, g.real != 0, g.kind)
+                    : all_ones_kind(g.kind)                                         ? pm1_waves(g.T, g.real != 0, g.loop != 0, g.kind)
                     : g.kind == KIND_REP                                            ? rep_waves(g.T, g.real != 0, g.H)
                                                                                     : 1;
             L.groups.push_back(gr);

@@ -310,7 +310,7 @@ __device__ __forceinline__ void epis_step(int kc, int st, EpiS<T>& E, const doub
         const int ko = a == 1 ? 1 : 2 * a - 1, ke = a == 1 ? 2 : 2 * a;
         double* dst = k < 2 ? &pt[ko].re : k < 4 ? &pt[ke].re : k < 6 ? &lv[2 * a].re : &lv[2 * a + 1].re;
         dst += k & 1;
-        if ((lane & 7) != 0) dst = &pt[GeoS<T>::PTN - 1].re;
+        if ((lane & 7) != 0 || (!LOOP && k >= 4)) dst = &pt[GeoS<T>::PTN - 1].re;  // no v slots
         *dst = E.s3;
     }
 }
@@ -573,18 +573,19 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     const int h = J.E;
 
     // ---- LDS carve (wh_pm1s_lds_bytes): X A, one V per wave (doubles for the real instance), D,
-    // per wave NB term slots (pt, lv)
-    // and z rows x 2
+    // per wave NB term slots (pt, lv) and z rows x 2 (D, lv and the z rows for loop hafnians only)
     constexpr int PTN = GeoS<T>::PTN;
     constexpr int NB = GeoS<T>::NB;
     cplx* Ms = reinterpret_cast<cplx*>(smem);
     const size_t mat = ((size_t)NP * S * (RE ? sizeof(double) : sizeof(cplx)) + 31) / 32 * 32 / sizeof(cplx);
     cplx* Pw = Ms + mat + wave * mat;
     cplx* Dv = Ms + (1 + nw) * mat;
-    constexpr int SMALL = NB * 2 * PTN + 2 * NP;
-    cplx* slots = Dv + NP + wave * SMALL;
-    cplx* Zb = slots + NB * 2 * PTN;  // z_a by row, two buffers (a even / odd)
-    double4* wres = reinterpret_cast<double4*>(Dv + NP + nw * SMALL);
+    constexpr int DVN = LOOP ? NP : 0;  // D and the z rows only for loop hafnians
+    constexpr int SL = LOOP ? 2 * PTN : PTN;  // a term slot: p_i, and v_i for loop hafnians
+    constexpr int SMALL = NB * SL + (LOOP ? 2 * NP : 0);
+    cplx* slots = Dv + DVN + wave * SMALL;
+    cplx* Zb = slots + NB * SL;  // z_a by row, two buffers (a even / odd)
+    double4* wres = reinterpret_cast<double4*>(Dv + DVN + nw * SMALL);
 
     // ---- stage X A (row r holds A's row swap(r)), zero padded, and D
     const cplx* A = pool + J.a_off;
@@ -613,8 +614,8 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
     int64_t t_batch = t0 + wave;  // the batch's first term
     for (int64_t t = t0 + wave; active && t < t1; t += nw) {
         const uint64_t tu = (uint64_t)t;
-        cplx* pt = slots + slot * 2 * PTN;
-        cplx* lv = pt + PTN;
+        cplx* pt = slots + slot * SL;
+        cplx* lv = pt + PTN;  // (loop hafnians only)
         // the lane's signs: bit p for row rd(p), bit T + q for column cd(q)
         uint32_t fm = 0;
 #pragma unroll
@@ -677,10 +678,10 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
             wave_lds_sync();  // pt and lv of every slot as the epilogues stored them
             if (lane < slot) {
                 // coefficient of eta^h in exp(sum_i a_i eta^i), a_i = p_i / 2i (+ v_i / 2)
-                const cplx* ptj = slots + lane * 2 * PTN;
+                const cplx* ptj = slots + lane * SL;
                 const uint64_t tj = (uint64_t)(t_batch + (int64_t)lane * nw);
                 const double prefac = ((h - __popcll(tj)) & 1) ? -1.0 : 1.0;
-                const cplx top = coeff_top_s<4 * T, LOOP, RE>(ptj, ptj + PTN, ldc<RE>(ptj + PTN + 1), h);
+                const cplx top = coeff_top_s<4 * T, LOOP, RE>(ptj, ptj + PTN, LOOP ? ldc<RE>(ptj + PTN + 1) : cmk(0.0, 0.0), h);
                 acc_sum.add(cmulr(top, prefac));
             }
             wave_lds_sync();  // the slots are free again
@@ -717,8 +718,10 @@ __device__ __forceinline__ void pm1s_body(const wh_job_dev& J, const cplx* __res
 
 }  // namespace
 
+// The real plain instance at T = 3 (n = 17..24) fits 128 registers and, without the loop
+// vectors and slots, 33 KB of LDS per 4-wave workgroup: four waves per SIMD instead of three.
 template <int T, bool LOOP, bool RE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((RE && T == 3 && !LOOP) ? 4 : 1)))
 wh_sieve_pm1s(const wh_job_dev* __restrict__ jobs, int njobs, const cplx* __restrict__ pool,
               double4* __restrict__ partials, int64_t block_offset) {
     extern __shared__ __align__(16) unsigned char wh_smem[];
@@ -806,18 +809,19 @@ wh_sieve_pm1s_persistent(const wh_job_dev* __restrict__ jobs, const cplx* __rest
 
 __global__ void wh_clear_counters(unsigned* __restrict__ counters) { counters[threadIdx.x] = 0u; }
 
-size_t wh_pm1s_lds_bytes(int T, int nw, bool real) {
+size_t wh_pm1s_lds_bytes(int T, int nw, bool real, bool loop) {
     size_t NP = 8 * (size_t)T, S = NP + WH_PM1S_PAD;
     size_t PTN = 4 * (size_t)T + 4, NB = T >= 6 ? 1 : T == 5 ? 7 : 8;
     size_t mat = (NP * S * (real ? sizeof(double) : sizeof(cplx)) + 31) / 32 * 32;
-    size_t small = (NB * 2 * PTN + 2 * NP) * sizeof(cplx);
-    return (1 + nw) * mat + NP * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
+    size_t dvn = loop ? NP : 0, sl = loop ? 2 * PTN : PTN;
+    size_t small = (NB * sl + 2 * dvn) * sizeof(cplx);
+    return (1 + nw) * mat + dvn * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
 
 template <int T, bool LOOP, bool RE>
 static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
-    size_t lds = wh_pm1s_lds_bytes(T, nw, RE);
+    size_t lds = wh_pm1s_lds_bytes(T, nw, RE, LOOP);
     static wh_lds_limit lim;
     hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lim);
     if (err != hipSuccess) return err;
@@ -837,7 +841,7 @@ hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, 
     for (int k = 0; k < ngroups; ++k) {
         items.g[k] = groups[k];
         if (groups[k].T < 1 || groups[k].T > 6 || groups[k].nw < 1 || groups[k].nw > 4) return hipErrorInvalidValue;
-        lds = std::max(lds, wh_pm1s_lds_bytes(groups[k].T, groups[k].nw, false));
+        lds = std::max(lds, wh_pm1s_lds_bytes(groups[k].T, groups[k].nw, false, true));
     }
     static wh_lds_limit lim;
     hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s_persistent, lds, lim);
