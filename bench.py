@@ -527,13 +527,24 @@ def dominant_kernel(step0, B, torch, stream):
 
 
 def kernel_probe(what, B, torch, stream, reps=5):
-    """Launch sequences for kernel traces: the repeated-row batch of configuration 4 on the
+    """Launch sequences for kernel traces: the real n = 24 batch of configuration 2 (config2), the
+    repeated-row batch of configuration 4 on the
     default route (tile sides 3 and 1), with every request at tile side 2 (2E <= 16, one term per
     wave) and on the general kernel (config4), or the first 1000 requests of the stream as
     batches of 500 with wh_tune_persistent 0, 1 and 2 (persistent); wall seconds per pass."""
     from thewalrus_amd import _native
 
     out = {}
+    if what == "config2":
+        b = B.DeviceBatch([B.prepare(M) for M in W.config2()])
+        b.run(stream)
+        ev, wall = timed_runs(b, reps, torch, stream)
+        vals = b.results()
+        out["default"] = {"kernel_ms": ev / reps * 1e3, "hafnians_per_s": reps * len(vals) / wall,
+                          "symmetric_tflops": b.flops / (ev / reps) / 1e12}
+        out["ones_exact"] = bool(round(vals[-1].real) == 316234143225)
+        b.close()
+        return out
     if what == "config4":
         preps = [B.prepare_repeated(A, rpt, mu=mu, loop=True) for A, rpt, mu in W.config4()]
         vals = {}
@@ -581,8 +592,9 @@ def main():
     ap.add_argument("--no-next", action="store_true", help="skip the batched loop hafnian / montrealer lines")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the dominant kernel's launches (for counter runs under rocprofv3)")
-    ap.add_argument("--only", choices=["config4", "persistent"],
-                    help="for kernel traces under rocprofv3: config4 = the repeated-row batch on the "
+    ap.add_argument("--only", choices=["config2", "config4", "persistent"],
+                    help="for kernel traces under rocprofv3: config2 = the real n = 24 batch on the "
+                    "default route; config4 = the repeated-row batch on the "
                     "default route and on the general kernel; persistent = 1000 stream requests as "
                     "batches of 500 with the persistent launch off, on (default rule) and forced")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
