@@ -368,11 +368,15 @@ int rep_waves(int T, bool real, int H) {
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
 // under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
+// The symmetric kernel's choice uses the loop carve also for plain launches: the estimate
+// ignores registers, and with the smaller plain carve it picks 3-wave workgroups at T = 4,
+// which the 300-register instance cannot place two of on a CU (n = 32: 10.7 -> 14.4 ms).
 int pm1_waves(int T, bool real, bool loop, int kind) {
+    (void)loop;
     const size_t lds_cap = 160 * 1024;
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
-        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real, loop)
+        size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real, true)
                    : kind == KIND_PM1M ? wh_pm1m_lds_bytes(T, nw)
                                        : wh_pm1_lds_bytes(T, real, nw);
         if (b > lds_cap) break;
