@@ -367,23 +367,40 @@ int rep_waves(int T, bool real, int H) {
 }
 
 // Waves per workgroup for a PM1 group: the count that maximises resident waves per CU
-// under the LDS budget (each wave owns one n_pad x n_pad power; the workgroup shares M).
-// The symmetric kernel's choice uses the loop carve also for plain launches: the estimate
-// ignores registers, and with the smaller plain carve it picks 3-wave workgroups at T = 4,
-// which the 300-register instance cannot place two of on a CU (n = 32: 10.7 -> 14.4 ms).
+// (each wave owns one n_pad x n_pad power; the workgroup shares M).  For the symmetric kernel
+// the runtime's occupancy calculator gives the workgroups per CU of each count (registers and
+// LDS: at T = 4 the 300-register instances hold one wave per SIMD, so 3-wave workgroups would
+// leave a SIMD idle; n = 32: 14.4 ms against 10.8 at 4 waves); without a device it falls back to
+// the LDS budget of the loop carve.  Ties go to the larger workgroup: the same waves per CU with
+// fewer staging copies of the matrix (the real T = 3 instance: 33.1 -> 30.5 ms on 4096 n = 24).
+std::atomic<int> g_pm1s_resident[kPm1sMaxT + 1][2][2][5];  // 1 + workgroups per CU, 0 = not asked yet
+
+int pm1s_resident_cached(int T, bool real, bool loop, int nw) {
+    std::atomic<int>& c = g_pm1s_resident[T][real ? 1 : 0][loop ? 1 : 0][nw];
+    int v = c.load(std::memory_order_relaxed);
+    if (v == 0) {
+        v = 1 + std::max(0, wh_pm1s_resident(T, loop, real, nw));
+        c.store(v, std::memory_order_relaxed);
+    }
+    return v - 1;
+}
+
 int pm1_waves(int T, bool real, bool loop, int kind) {
-    (void)loop;
     const size_t lds_cap = 160 * 1024;
+    int res[5] = {0, 0, 0, 0, 0};
+    bool known = kind == KIND_PM1S && T >= 1 && T <= kPm1sMaxT;
+    for (int nw = 1; known && nw <= 4; ++nw) {
+        res[nw] = pm1s_resident_cached(T, real, loop, nw);
+        if (res[nw] <= 0) known = false;
+    }
     int best = 1, best_w = 0;
     for (int nw = 1; nw <= 4; ++nw) {  // kernel launch bound: 256 threads
         size_t b = kind == KIND_PM1S   ? wh_pm1s_lds_bytes(T, nw, real, true)
                    : kind == KIND_PM1M ? wh_pm1m_lds_bytes(T, nw)
                                        : wh_pm1_lds_bytes(T, real, nw);
-        if (b > lds_cap) break;
-        int per_cu = (int)(lds_cap / b) * nw;
+        if (!known && b > lds_cap) break;
+        int per_cu = known ? res[nw] * nw : (int)(lds_cap / b) * nw;
         if (per_cu > 16) per_cu = 16;
-        // ties go to the larger workgroup: the same waves per CU with fewer staging copies of
-        // the matrix (the real symmetric instance at T = 3: 33.1 -> 30.5 ms on 4096 n = 24)
         if (per_cu >= best_w) { best_w = per_cu; best = nw; }
     }
     return best;

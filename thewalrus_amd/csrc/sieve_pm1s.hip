@@ -818,12 +818,19 @@ size_t wh_pm1s_lds_bytes(int T, int nw, bool real, bool loop) {
     return (1 + nw) * mat + dvn * sizeof(cplx) + nw * small + nw * sizeof(double4) + 32;
 }
 
+// the dynamic-LDS limit set on each instance (shared by its launches and occupancy queries, so
+// that a query for a smaller workgroup never lowers what a launch relies on)
+template <int T, bool LOOP, bool RE>
+static wh_lds_limit& lds_limit_s() {
+    static wh_lds_limit lim;
+    return lim;
+}
+
 template <int T, bool LOOP, bool RE>
 static hipError_t launch_s(const wh_job_dev* jobs, int njobs, const void* pool, void* partials, int64_t nblocks,
                            int64_t block_offset, int nw, hipStream_t stream) {
     size_t lds = wh_pm1s_lds_bytes(T, nw, RE, LOOP);
-    static wh_lds_limit lim;
-    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lim);
+    hipError_t err = wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lds_limit_s<T, LOOP, RE>());
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL((wh_sieve_pm1s<T, LOOP, RE>), dim3((unsigned)nblocks), dim3(nw * WH_WAVE), lds, stream,
                        jobs, njobs, (const cplx*)pool, (double4*)partials, block_offset);
@@ -850,6 +857,45 @@ hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, 
     hipLaunchKernelGGL(wh_sieve_pm1s_persistent, dim3((unsigned)nwg), dim3(256), lds, stream, jobs,
                        (const cplx*)pool, (double4*)partials, counters, (double2*)out, items);
     return hipGetLastError();
+}
+
+template <int T, bool LOOP, bool RE>
+static int resident_s(int nw) {
+    int blocks = 0;
+    const size_t lds = wh_pm1s_lds_bytes(T, nw, RE, LOOP);
+    if (lds > 160 * 1024) return 0;  // the CU's LDS: no workgroup of this size fits
+    // a failed query leaves no error behind for the caller's next HIP call to report
+    if (wh_raise_lds_limit((const void*)wh_sieve_pm1s<T, LOOP, RE>, lds, lds_limit_s<T, LOOP, RE>()) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)wh_sieve_pm1s<T, LOOP, RE>, nw * WH_WAVE,
+                                                     lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return blocks;
+}
+
+int wh_pm1s_resident(int T, bool loop, bool real, int nw) {
+    if (nw < 1 || nw > 4) return 0;
+#define WH_PM1S_RES(TT)                                                                        \
+    case TT:                                                                                   \
+        return loop ? resident_s<TT, true, false>(nw) : resident_s<TT, false, false>(nw);
+#define WH_PM1S_RES_R(TT)                                                                      \
+    case TT:                                                                                   \
+        if (real) return loop ? resident_s<TT, true, true>(nw) : resident_s<TT, false, true>(nw); \
+        return loop ? resident_s<TT, true, false>(nw) : resident_s<TT, false, false>(nw);
+    switch (T) {
+        WH_PM1S_RES_R(1)
+        WH_PM1S_RES_R(2)
+        WH_PM1S_RES_R(3)
+        WH_PM1S_RES_R(4)
+        WH_PM1S_RES_R(5)
+        WH_PM1S_RES(6)
+        WH_PM1S_RES(7)
+        WH_PM1S_RES(8)
+        default: return 0;
+    }
+#undef WH_PM1S_RES
+#undef WH_PM1S_RES_R
 }
 
 hipError_t wh_launch_pm1s(int T, bool loop, bool real, const wh_job_dev* jobs, int njobs, const void* pool,
