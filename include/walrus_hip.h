@@ -322,8 +322,9 @@ int wh_tune_set(int kind, int T, int loop, int64_t blocks_per_request);
 int64_t wh_tune_get(int kind, int T, int loop);
 
 /* Waves per workgroup for an all-ones (kind 0, T = 1..6) or repeated-row (kind 2, T = 1..3)
- * kernel class, 1..4, or 0 for the default (the count that fits the most waves per CU in the
- * LDS); a value whose workgroup would not fit the LDS is ignored.  Set by the autotuner; results
+ * kernel class, 1..4, or 0 for the default (the count that fits the most waves per CU: by the
+ * runtime's occupancy calculator on the symmetric matrix-core kernel, by the LDS elsewhere); a
+ * value whose workgroup would not fit the LDS is ignored.  Set by the autotuner; results
  * change in their last bits at most (the order in which a chunk's terms are summed). */
 int wh_tune_set_waves(int kind, int T, int loop, int waves);
 
