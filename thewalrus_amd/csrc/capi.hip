@@ -373,7 +373,9 @@ int rep_waves(int T, bool real, int H) {
 // leave a SIMD idle; n = 32: 14.4 ms against 10.8 at 4 waves); without a device it falls back to
 // the LDS budget of the loop carve.  Ties go to the larger workgroup: the same waves per CU with
 // fewer staging copies of the matrix (the real T = 3 instance: 33.1 -> 30.5 ms on 4096 n = 24).
-std::atomic<int> g_pm1s_resident[kPm1sMaxT + 1][2][2][5];  // 1 + workgroups per CU, 0 = not asked yet
+// 1 + workgroups per CU, 0 = not asked yet; asked once per process on the current device (the
+// answer depends on the kernel's registers and the CU's LDS, alike on the GPUs of one host)
+std::atomic<int> g_pm1s_resident[kPm1sMaxT + 1][2][2][5];
 
 int pm1s_resident_cached(int T, bool real, bool loop, int nw) {
     std::atomic<int>& c = g_pm1s_resident[T][real ? 1 : 0][loop ? 1 : 0][nw];
