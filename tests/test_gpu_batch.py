@@ -444,6 +444,32 @@ def test_forced_tile_sides_and_waves(loop):
         db.close()
 
 
+@pytest.mark.parametrize("real", [False, True])
+def test_default_waves_fill_the_cu_at_tile_side_4(real):
+    """The symmetric kernel's default waves per workgroup come from the occupancy calculator: at
+    T = 4 (n = 25..32) the complex instances hold one wave per SIMD and the real ones two, so
+    the default is four-wave workgroups either way (an estimate from the LDS alone chose three
+    for the smaller plain carve and for the real instances, leaving SIMD slots empty).  The
+    default batch has the same chunking as waves=4 (same block count, bit-identical values), and
+    both match truth."""
+    rng = np.random.default_rng(97 + real)
+    reqs = [(random_symmetric(rng, n, complex_=not real, scale=1 / np.sqrt(n)), loop)
+            for n, loop in ((26, False), (26, True), (28, False))]
+    preps = [B.prepare(A, loop=l) for A, l in reqs]
+    truths = [O.hafnian_sieve(A, loop=l, precision="dd") for A, l in reqs]
+    out = {}
+    for w in (0, 4):
+        print("T = 4 default waves: real=%s waves=%d" % (real, w), flush=True)
+        db = B.DeviceBatch(preps, waves=w)
+        db.run()
+        out[w] = (db.blocks, db.results())
+        db.close()
+    assert out[0][0] == out[4][0], (out[0][0], out[4][0])
+    assert np.array_equal(out[0][1], out[4][1])
+    for (A, l), r, t in zip(reqs, out[0][1], truths):
+        assert relerr(r, t) < TOL, (A.shape[0], l, relerr(r, t))
+
+
 def test_tuned_tile_side_equals_the_tile_option():
     """A tile side set in the tuning table (wh_tune_set_tile) runs its class's requests exactly as
     the request option WH_OPT_TILE does (bit-identical), and a request's own option wins."""
