@@ -268,20 +268,23 @@ def test_tuned_chunking_changes_chunks_not_results():
     assert ra[1] == rb[1]
 
 
+@pytest.mark.parametrize("real", [False, True])
 @pytest.mark.parametrize("loop", [False, True])
-def test_odd_chunk_sizes_agree(loop):
+def test_odd_chunk_sizes_agree(loop, real):
     """Chunks of 1, 3, 5 and 13 terms (waves of a workgroup with unequal term counts, partial
-    coefficient batches) give the default chunking's results to rounding, on the symmetric
-    matrix-core kernel and on the vector kernel."""
-    rng = np.random.default_rng(81 + loop)
-    reqs = [(random_symmetric(rng, n, scale=1 / np.sqrt(n)), loop) for n in (10, 16, 22, 26)]
+    coefficient batches) and of 128 (several full coefficient batches per wave, the size that
+    large uniform batches may pass) give the default chunking's results to rounding, on the
+    symmetric matrix-core kernel (complex, and real: its real-only instances) and on the vector
+    kernel."""
+    rng = np.random.default_rng(81 + loop + 2 * real)
+    reqs = [(random_symmetric(rng, n, complex_=not real, scale=1 / np.sqrt(n)), loop) for n in (10, 16, 22, 26)]
     preps = [B.prepare(A, loop=l) for A, l in reqs]
     for kw in ({}, {"valu_products": True}):
         ref = B.DeviceBatch(preps, **kw)
         ref.run()
         r0 = ref.results()
         ref.close()
-        for ct in (1, 3, 5, 13):
+        for ct in (1, 3, 5, 13, 128):
             db = B.DeviceBatch(preps, chunk_terms=ct, **kw)
             db.run()
             r = db.results()
