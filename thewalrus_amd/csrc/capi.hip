@@ -377,11 +377,11 @@ int rep_waves(int T, bool real, int H) {
 // answer depends on the kernel's registers and the CU's LDS, alike on the GPUs of one host)
 std::atomic<int> g_pm1s_resident[kPm1sMaxT + 1][2][2][5];
 
-int pm1s_resident_cached(int T, bool real, bool loop, int nw) {
+int pm1s_resident_cached(int T, int nw, bool real, bool loop) {
     std::atomic<int>& c = g_pm1s_resident[T][real ? 1 : 0][loop ? 1 : 0][nw];
     int v = c.load(std::memory_order_relaxed);
     if (v == 0) {
-        v = 1 + std::max(0, wh_pm1s_resident(T, loop, real, nw));
+        v = 1 + std::max(0, wh_pm1s_resident(T, nw, real, loop));
         c.store(v, std::memory_order_relaxed);
     }
     return v - 1;
@@ -392,7 +392,7 @@ int pm1_waves(int T, bool real, bool loop, int kind) {
     int res[5] = {0, 0, 0, 0, 0};
     bool known = kind == KIND_PM1S && T >= 1 && T <= kPm1sMaxT;
     for (int nw = 1; known && nw <= 4; ++nw) {
-        res[nw] = pm1s_resident_cached(T, real, loop, nw);
+        res[nw] = pm1s_resident_cached(T, nw, real, loop);
         if (res[nw] <= 0) known = false;
     }
     int best = 1, best_w = 0;

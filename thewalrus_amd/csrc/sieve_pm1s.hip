@@ -874,7 +874,7 @@ static int resident_s(int nw) {
     return blocks;
 }
 
-int wh_pm1s_resident(int T, bool loop, bool real, int nw) {
+int wh_pm1s_resident(int T, int nw, bool real, bool loop) {
     if (nw < 1 || nw > 4) return 0;
 #define WH_PM1S_RES(TT)                                                                        \
     case TT:                                                                                   \

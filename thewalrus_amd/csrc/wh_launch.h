@@ -43,7 +43,7 @@ hipError_t wh_launch_pm1m(int T, bool loop, bool mtl, const wh_job_dev* jobs, in
 size_t wh_pm1s_lds_bytes(int T, int nw, bool real, bool loop);
 // Workgroups of nw waves that one CU holds for that instance (registers, LDS, wave slots), from
 // the runtime's occupancy calculator on the current device; 0 when it cannot say.
-int wh_pm1s_resident(int T, bool loop, bool real, int nw);
+int wh_pm1s_resident(int T, int nw, bool real, bool loop);
 struct wh_mixed_group;
 hipError_t wh_launch_pm1s_persistent(const wh_mixed_group* groups, int ngroups, const wh_job_dev* jobs, int njobs,
                                      const void* pool, void* partials, unsigned* counters, void* out, int nwg,
