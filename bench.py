@@ -527,7 +527,8 @@ def dominant_kernel(step0, B, torch, stream):
 
 
 def kernel_probe(what, B, torch, stream, reps=5):
-    """Launch sequences for kernel traces: the real n = 24 batch of configuration 2 (config2), the
+    """Launch sequences for kernel traces: real n = 32 and n = 40 batches at the default waves
+    per workgroup and at 3 (real_waves), the real n = 24 batch of configuration 2 (config2), the
     repeated-row batch of configuration 4 on the
     default route (tile sides 3 and 1), with every request at tile side 2 (2E <= 16, one term per
     wave) and on the general kernel (config4), or the first 1000 requests of the stream as
@@ -535,6 +536,20 @@ def kernel_probe(what, B, torch, stream, reps=5):
     from thewalrus_amd import _native
 
     out = {}
+    if what == "real_waves":
+        rng = np.random.default_rng(7)
+        for n, loop, cnt in ((32, False, 16), (32, True, 16), (40, False, 1), (40, True, 1)):
+            mats = [rng.uniform(-1, 1, size=(n, n)) for _ in range(cnt)]
+            preps = [B.prepare((M + M.T) / 2, loop=loop) for M in mats]
+            row = {}
+            for w in (0, 3):
+                b = B.DeviceBatch(preps, waves=w)
+                b.run(stream)
+                ev, wall = timed_runs(b, reps, torch, stream)
+                row["waves_%d" % w] = {"kernel_ms": ev / reps * 1e3, "blocks": b.blocks}
+                b.close()
+            out["real_n%d_%s_x%d" % (n, "loop" if loop else "plain", cnt)] = row
+        return out
     if what == "config2":
         b = B.DeviceBatch([B.prepare(M) for M in W.config2()])
         b.run(stream)
@@ -592,8 +607,9 @@ def main():
     ap.add_argument("--no-next", action="store_true", help="skip the batched loop hafnian / montrealer lines")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the dominant kernel's launches (for counter runs under rocprofv3)")
-    ap.add_argument("--only", choices=["config2", "config4", "persistent"],
-                    help="for kernel traces under rocprofv3: config2 = the real n = 24 batch on the "
+    ap.add_argument("--only", choices=["config2", "config4", "persistent", "real_waves"],
+                    help="for kernel traces under rocprofv3: real_waves = real n = 32 and 40 batches "
+                    "with the default waves per workgroup and with 3; config2 = the real n = 24 batch on the "
                     "default route; config4 = the repeated-row batch on the "
                     "default route and on the general kernel; persistent = 1000 stream requests as "
                     "batches of 500 with the persistent launch off, on (default rule) and forced")
